@@ -1,0 +1,235 @@
+/*
+ * sw2d.h -- C ABI of the MI355X SW2D library (libsw2d.so).
+ *
+ * The reference (LabHidro/SW2D-EFDC, SW2D-EFDC.cu) has no library API: its
+ * SW2D time step lives inside main() and is driven through raw device
+ * pointers.  Each entry point below replaces one piece of that in-process
+ * call sequence; the comment on each names the reference code it replaces.
+ * Plain C: opaque handle, plain pointers and sizes, int status (0 = OK,
+ * negative = error, text from sw2d_last_error), no exceptions cross it.
+ *
+ * Grid convention (the reference's): rows*cols cells in row-major memory
+ * order, row 0 the north row of the ESRI grid, "um" the flux through the
+ * north face of a cell (positive southwards, row direction, spacing dx),
+ * "vn" the flux through its west face (positive eastwards, spacing dy).
+ * Every field array passed in or out holds rows*cols doubles.
+ *
+ * Threading: one host thread per handle.  Work is queued on the handle's own
+ * HIP stream; functions that return data synchronise that stream.
+ */
+#ifndef SW2D_H
+#define SW2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SW2D_ABI_VERSION 1
+
+/* status codes */
+#define SW2D_OK 0
+#define SW2D_E_ARG (-1)     /* bad argument */
+#define SW2D_E_HIP (-2)     /* HIP runtime error */
+#define SW2D_E_NOMEM (-3)   /* device allocation failed */
+#define SW2D_E_STATE (-4)   /* call not valid in this state */
+#define SW2D_E_IO (-5)      /* file missing or malformed */
+#define SW2D_E_NODEV (-6)   /* no usable GPU */
+#define SW2D_E_UNSUPPORTED (-7)
+
+/* arithmetic of the state and fluxes */
+#define SW2D_FP64 0   /* FP64 everywhere (parity mode) */
+#define SW2D_FP32 1   /* FP32 state/fluxes, FP64 dt, volume and hydrograph */
+
+/* kernel organisation */
+#define SW2D_KERNEL_FUSED 0  /* one fused, row-streaming kernel per iteration */
+#define SW2D_KERNEL_PLAIN 1  /* the reference's unfused kernel sequence */
+
+/* fields for sw2d_get_field */
+#define SW2D_FIELD_H 0     /* depth h */
+#define SW2D_FIELD_UM 1    /* row-direction face flux um */
+#define SW2D_FIELD_VN 2    /* column-direction face flux vn */
+#define SW2D_FIELD_UU 3    /* cell-centre velocity uu (VTK y_velocity = -uu) */
+#define SW2D_FIELD_VV 4    /* cell-centre velocity vv (VTK x_velocity) */
+#define SW2D_FIELD_UUA 5   /* uu1 averaged onto column faces */
+#define SW2D_FIELD_VVA 6   /* vv1 averaged onto row faces */
+#define SW2D_FIELD_Z 7     /* bed elevation as uploaded */
+#define SW2D_FIELD_HM 8    /* row-face depth */
+#define SW2D_FIELD_HN 9    /* column-face depth */
+#define SW2D_FIELD_UU1 10  /* row-face velocity */
+#define SW2D_FIELD_VV1 11  /* column-face velocity */
+
+typedef struct sw2d_handle sw2d_handle;
+
+/* dem.dat header (SW2D-EFDC.cu:3990-3995) */
+typedef struct sw2d_grid_desc {
+    int rows, cols;
+    double xllcorner, yllcorner, cellsize, nodata;
+} sw2d_grid_desc;
+
+/* scalars of setup.dat and the constants main() hard-codes */
+typedef struct sw2d_params {
+    double g;          /* gravity, 9.80 (SW2D-EFDC.cu:3977) */
+    double manning;    /* Manning n (setup.dat, 3789) */
+    double th;         /* dry threshold, 1.0e-4 m (4644) */
+    double dt;         /* TIME STEP, half an iteration (3771); dt2 = 2 dt (3976) */
+    double time0;      /* start time in s (4230) */
+    double INT, INF, LWL, EV_WL_min;  /* loss fractions, 3781-3787 */
+    int precision;     /* SW2D_FP64 or SW2D_FP32 */
+    int kernel;        /* SW2D_KERNEL_FUSED or SW2D_KERNEL_PLAIN */
+    int adaptive;      /* extension: CFL-adaptive iteration length (0 = fixed dt) */
+    double courant;    /* Courant number of the adaptive step (0.5) */
+    int device;        /* HIP device ordinal */
+    int tile_rows;     /* rows per tile of the fused kernel (0 = automatic) */
+} sw2d_params;
+
+/* moving Gaussian storm (extension): rate(x, y, t) = peak * gx * gy with
+ * gx = exp(-(x - (x0 + vx t))^2 / (2 sigma^2)), gy likewise, for
+ * t0 <= t < t1; x = (col + 0.5) cellsize, y = (row + 0.5) cellsize measured
+ * from the north-west corner; peak in m/s.  Replaces the rain series. */
+typedef struct sw2d_storm {
+    double peak, x0, y0, vx, vy, sigma, t0, t1;
+} sw2d_storm;
+
+/* volume ledger (extension), all volumes in m3 */
+typedef struct sw2d_balance_out {
+    double volume;       /* sum of h * dx * dy over active cells now */
+    double volume0;      /* same at creation */
+    double v_source;     /* rain + point sources - evaporation admitted */
+    double v_clip;       /* volume added by clipping negative depths to 0 */
+    double v_outflow;    /* volume through the outlet weir */
+    double q_out;        /* outlet discharge of the last iteration, m3/s */
+    double q_peak;       /* peak outlet discharge, m3/s */
+    double t_peak;       /* time of the peak, s */
+    double error;        /* volume - (volume0 + v_source + v_clip - v_outflow) */
+} sw2d_balance_out;
+
+/* Fill params with the reference's defaults (g 9.80, th 1e-4, fused FP64). */
+void sw2d_default_params(sw2d_params *p);
+
+/* Library/ABI version (SW2D_ABI_VERSION). */
+int sw2d_version(void);
+
+/* Text of the last error on h, or of the last failed sw2d_create if h is NULL. */
+const char *sw2d_last_error(const sw2d_handle *h);
+
+/* Copy bed z and initial depth h0 to the device and apply the reference's
+ * initialisation: init_inf + initiald (SW2D-EFDC.cu:29-91, launched at
+ * 4763-4766) after the uploads of 4677-4729.  Cells with z == nodata are
+ * inactive.  The library owns all device memory of the handle. */
+int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
+                const sw2d_params *params, sw2d_handle **out);
+
+/* Free the handle and its device memory (replaces cudaFree, 5126-5160). */
+int sw2d_destroy(sw2d_handle *h);
+
+/* Rain series in mm per interval dtrain (rain.dat, 4078-4108).  Evaluated on
+ * the device every half step with stream_flow's interpolation (546-553),
+ * replacing the host call + H2D copy of rr at 4820-4822 and 4843-4846. */
+int sw2d_set_rain_series(sw2d_handle *h, const double *rain_mm, int n, double dtrain);
+
+/* Point sources: nst coordinates in metres (coord_source_sink.dat,
+ * 4016-4023) and the Q table of nq rows x nst values in m3/s, spacing dtoq
+ * (Q_source_sink.dat, 4041-4069).  The cell index uses stream_flow's formula
+ * (541-544); values are evaluated on the device every half step (531-540),
+ * replacing the dense ql upload of 4821 and 4845.  A source outside the
+ * rows*cols real cells is an error. */
+int sw2d_set_sources(sw2d_handle *h, const double *x, const double *y, int nst,
+                     const double *Q, int nq, double dtoq);
+
+/* Evaporation series in mm per interval dtrain (the output of
+ * gpu_evaporation_calc, 556-586); enables E in continuity (626-633) through
+ * evaporation_load (588-600), replacing the H2D of Ev at 4824-4827. */
+int sw2d_set_evaporation(sw2d_handle *h, const double *evapo_mm, int n);
+
+/* evaporation_calc semantics on the host: fills evapo_mm[n] from albedo,
+ * temperatures T and solar radiation Rg (556-586). */
+int sw2d_evaporation_series(double albedo, const double *T, const double *Rg, int n,
+                            double dtime, double *evapo_mm);
+
+/* Extension: per-cell Manning class (rows*cols bytes) and its table of nclass
+ * (<= 8) coefficients; a face uses 0.5 * (n_west + n_east). */
+int sw2d_set_manning_map(sw2d_handle *h, const unsigned char *cls, const double *table,
+                         int nclass);
+
+/* Extension: moving Gaussian storm (replaces the rain series). */
+int sw2d_set_storm(sw2d_handle *h, const sw2d_storm *storm);
+
+/* Extension: free-outflow outlet at an edge cell; its edge face passes the
+ * weir discharge 0.5440 h sqrt(g h) of SW2D-EFDC.cu:131 (depth at the start
+ * of the iteration), zeroed like any face when the cell dries. */
+int sw2d_set_outlet(sw2d_handle *h, int row, int col);
+
+/* Extension: record the outlet hydrograph, up to cap iterations. */
+int sw2d_set_hydrograph(sw2d_handle *h, long cap);
+
+/* Extension: lake-inflow accumulation.  mask holds rows*cols ints (mask.dat,
+ * lake = 1); the records are those of couple_BC_efdc (728-1067) and each
+ * record's inflow rate (the volumeFlow_BC == 1 formulas, 764-821) is
+ * integrated over every iteration on the device. */
+int sw2d_set_lake_bc(sw2d_handle *h, const int *mask);
+
+/* Advance n loop iterations of the reference's call sequence (4798-4877):
+ * flux, continuity(t), treat_error, continuity(t + dt), derived fields.
+ * Asynchronous: returns once the work is queued. */
+int sw2d_step(sw2d_handle *h, int n_iters);
+
+/* Run iterations while time + dt <= tmax (the loop condition of 4798), at
+ * most max_iters; *done receives the number run.  Synchronises. */
+int sw2d_run_until(sw2d_handle *h, double tmax, long max_iters, long *done);
+
+/* Wait for queued work. */
+int sw2d_sync(sw2d_handle *h);
+
+/* Simulated time (s), the reference's mstep (2 per iteration) and the
+ * length dt2 of the next iteration.  Synchronises. */
+int sw2d_time(sw2d_handle *h, double *time, long *mstep, double *dt2_next);
+
+/* Copy one field (rows*cols doubles) to caller memory; derived fields are
+ * computed from the current state as hm_hn/uu1_vv1/uu_vv/uua_vva do
+ * (362-522).  Replaces the D2H copies at output times (4900, 4976, 5026,
+ * 5042, 5058).  Synchronises. */
+int sw2d_get_field(sw2d_handle *h, int field, double *out);
+
+/* couple_BC_efdc (728-1067) on caller-supplied host fields: records of the
+ * four edge lists (S, N, E, W in the reference's naming) with the EFDC
+ * (I, J) and inflow (m3/s).  Arrays cells/I/J/q hold 4*cap entries, list k
+ * at offset k*cap; counts[4] receives the list lengths.  h, vva, uua are
+ * rows*cols doubles (the reference passes h_vva as h_vn and h_uua as h_um,
+ * 4903/4909).  Host-only. */
+int sw2d_couple_bc(int rows, int cols, double dx, double dy, const int *mask, const double *h,
+                   const double *vva, const double *uua, int cap, int *cells, int *I, int *J,
+                   double *q, int *counts);
+
+/* lake_catchment_smooth (2766-2827) on a host DEM, in place.  Host-only. */
+int sw2d_lake_catchment_smooth(int rows, int cols, const int *mask, double *z);
+
+/* Volume ledger (extension).  Synchronises. */
+int sw2d_balance(sw2d_handle *h, sw2d_balance_out *out);
+
+/* Outlet hydrograph: up to cap (time, Q) pairs; *n receives the count. */
+int sw2d_hydrograph(sw2d_handle *h, double *out, long cap, long *n);
+
+/* Accumulated lake inflow volume per record (order of sw2d_set_lake_bc:
+ * all S records, then N, E, W); *n receives the record count. */
+int sw2d_lake_inflow(sw2d_handle *h, double *out, int cap, int *n);
+
+/* Timing of the main step kernel with HIP events on the handle's stream:
+ * enable (1) or disable (0); sw2d_kernel_time returns the summed kernel
+ * milliseconds and launch count since enabling.  Synchronises. */
+int sw2d_timing(sw2d_handle *h, int enable);
+int sw2d_kernel_time(sw2d_handle *h, double *ms, long *launches);
+
+/* Device memory the handle holds, in bytes. */
+int sw2d_device_bytes(sw2d_handle *h, long long *bytes);
+
+/* Evaluate the library's device math on n inputs (diagnostics and tests):
+ * fn 0 pow(x, 1.3333330), 1 exp, 2 log, 3 sqrt, 4 division x/y,
+ * 5 fmax(x, y), 6 the rain interpolation of stream_flow with y = time.
+ * x, y, out hold n doubles (y may be NULL where unused). */
+int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SW2D_H */

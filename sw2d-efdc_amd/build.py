@@ -1,0 +1,62 @@
+"""Build libsw2d.so (HIP, gfx950) and the sw2d-efdc driver in place.
+
+Run as a script or call build().  hipcc cross-compiles for gfx950 without a
+GPU.  Flags:
+  -ffp-contract=off  no fused multiply-add: FP64 results follow the written
+                     evaluation order, as the CPU FP64 check does;
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                     no packed FP32 instructions.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libsw2d.so")
+DRIVER = os.path.join(HERE, "sw2d-efdc")
+
+HIP_SOURCES = ["sw2d_api.hip", "sw2d_fused.hip", "sw2d_plain.hip", "sw2d_host.cpp"]
+DRIVER_SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp"]
+ARCH = os.environ.get("SW2D_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
+          "-Wno-unused-function", "-Wno-unused-variable",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(verbose=False):
+    """Compile every HIP/C++ source of the package (always recompiles)."""
+    env = dict(os.environ)
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp, *srcs]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True, env=env)
+    os.replace(tmp, LIB)
+    dsrcs = [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
+    if all(os.path.exists(s) for s in dsrcs):
+        tmp = DRIVER + ".tmp"
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, *dsrcs,
+               f"-L{HERE}", "-lsw2d", f"-Wl,-rpath,$ORIGIN"]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.run(cmd, check=True, env=env)
+        os.replace(tmp, DRIVER)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    print(LIB)

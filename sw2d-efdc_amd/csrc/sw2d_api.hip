@@ -1,0 +1,749 @@
+// sw2d_api.hip -- host implementation of the C ABI declared in include/sw2d.h.
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "sw2d_common.hpp"
+
+using namespace sw2d;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+int fail(sw2d_handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int hip_fail(sw2d_handle *h, hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, SW2D_E_HIP, m);
+}
+
+#define HIPCHK(h, expr)                                   \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+
+template <typename T>
+int dalloc(sw2d_handle *h, T **p, size_t count)
+{
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess) return fail(h, SW2D_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemsetAsync(q, 0, bytes, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemsetAsync");
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return SW2D_OK;
+}
+
+void dfree(sw2d_handle *h, void *p)
+{
+    if (!p) return;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    (void)hipFree(p);
+}
+
+int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// the StepCommon of iteration k = h->step_index
+StepCommon make_common(sw2d_handle *h)
+{
+    StepCommon s{};
+    const long long k = h->step_index;
+    s.R = h->R; s.C = h->C;
+    s.dx = h->grid.cellsize; s.dy = h->grid.cellsize;
+    s.g = h->prm.g; s.rn = h->prm.manning; s.th = h->prm.th;
+    s.P_lake = 1.00 - h->prm.LWL;
+    s.P_land = 1.00 - (h->prm.INT + h->prm.INF);
+    s.EV_WL_min = h->prm.EV_WL_min;
+    s.fixed_dt = !h->prm.adaptive;
+    s.dt_fixed = h->prm.dt;
+    s.dt2_fixed = 2.00 * h->prm.dt;
+    s.courant = h->prm.courant;
+    s.dt2_max = 2.00 * h->prm.dt;
+    s.clk = h->clk;
+    s.slot_a = (int)(k % 3); s.slot_b = (int)((k + 1) % 3); s.slot_c = (int)((k + 2) % 3);
+    s.step_index = k;
+    s.rain = h->rain; s.nrain = h->nrain; s.dtrain = h->dtrain;
+    s.evapo = h->evapo; s.nevapo = h->nevapo; s.evap_on = h->evap_on;
+    s.Q = h->Q; s.nq = h->nq; s.nst = h->nst; s.dtoq = h->dtoq;
+    s.src_r = h->src_r; s.src_c = h->src_c;
+    s.storm_on = h->storm_on;
+    s.st_peak = h->storm.peak; s.st_x0 = h->storm.x0; s.st_y0 = h->storm.y0;
+    s.st_vx = h->storm.vx; s.st_vy = h->storm.vy; s.st_sigma = h->storm.sigma;
+    s.st_t0 = h->storm.t0; s.st_t1 = h->storm.t1;
+    s.ncls_on = h->ncls_on;
+    for (int i = 0; i < kMaxClasses; i++) s.ntab[i] = h->ntab[i];
+    s.out_r = h->out_r; s.out_c = h->out_c; s.out_edge = h->out_edge;
+    s.part_out = h->part + (size_t)(k % 2) * h->nparts;
+    s.part_in = h->part + (size_t)((k + 1) % 2) * h->nparts;
+    s.nparts = h->nparts;
+    s.ledger = h->ledger;
+    s.hydro = h->hydro; s.hydro_cap = h->hydro_cap;
+    s.nbc = h->nbc; s.bc_r = h->bc_r; s.bc_c = h->bc_c; s.bc_kind = h->bc_kind; s.bc_acc = h->bc_acc;
+    return s;
+}
+
+bool fused(const sw2d_handle *h) { return h->prm.kernel == SW2D_KERNEL_FUSED; }
+
+int choose_tile_rows(const sw2d_handle *h)
+{
+    if (h->prm.tile_rows > 0) return std::min(h->prm.tile_rows, 512);
+    // enough tiles to give every CU several waves, tiles tall enough that the
+    // 6 warm-up rows stay a small overhead
+    const int nstrips = (h->C + 57) / 58;
+    int H = 128;
+    while (H > 32 && (long)nstrips * ((h->R + H - 1) / H) < 4096) H /= 2;
+    return H;
+}
+
+int upload_source_cells(sw2d_handle *h, const int *rr, const int *cc, int n)
+{
+    dfree(h, h->src_r); dfree(h, h->src_c);
+    h->src_r = h->src_c = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &h->src_r, n)) || (rc = dalloc(h, &h->src_c, n))) return rc;
+    if (n > 0) {
+        HIPCHK(h, hipMemcpy(h->src_r, rr, sizeof(int) * n, hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->src_c, cc, sizeof(int) * n, hipMemcpyHostToDevice));
+    }
+    return SW2D_OK;
+}
+
+int check_idle(sw2d_handle *h)
+{
+    if (!h) return SW2D_E_ARG;
+    return SW2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void sw2d_default_params(sw2d_params *p)
+{
+    if (!p) return;
+    memset(p, 0, sizeof(*p));
+    p->g = 9.80;
+    p->manning = 0.03;
+    p->th = 1.0e-4;
+    p->dt = 0.5;
+    p->precision = SW2D_FP64;
+    p->kernel = SW2D_KERNEL_FUSED;
+    p->courant = 0.5;
+}
+
+int sw2d_version(void) { return SW2D_ABI_VERSION; }
+
+const char *sw2d_last_error(const sw2d_handle *h)
+{
+    return h ? h->err.c_str() : g_create_error.c_str();
+}
+
+int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
+                const sw2d_params *params, sw2d_handle **out)
+{
+    if (!out) return fail(nullptr, SW2D_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (!grid || !z || !h0 || !params) return fail(nullptr, SW2D_E_ARG, "NULL argument");
+    if (grid->rows < 2 || grid->cols < 2) return fail(nullptr, SW2D_E_ARG, "grid must be at least 2x2");
+    if (!(grid->cellsize > 0)) return fail(nullptr, SW2D_E_ARG, "cellsize must be positive");
+    if (params->precision != SW2D_FP64 && params->precision != SW2D_FP32)
+        return fail(nullptr, SW2D_E_ARG, "unknown precision");
+    if (params->kernel != SW2D_KERNEL_FUSED && params->kernel != SW2D_KERNEL_PLAIN)
+        return fail(nullptr, SW2D_E_ARG, "unknown kernel");
+    if (params->kernel == SW2D_KERNEL_PLAIN && params->precision != SW2D_FP64)
+        return fail(nullptr, SW2D_E_UNSUPPORTED, "the plain kernel sequence is FP64 only");
+    if (!(params->dt > 0)) return fail(nullptr, SW2D_E_ARG, "dt must be positive");
+    if (params->adaptive && !(params->courant > 0)) return fail(nullptr, SW2D_E_ARG, "courant must be positive");
+    if (params->tile_rows < 0 || params->tile_rows > 512)
+        return fail(nullptr, SW2D_E_ARG, "tile_rows must be in [0, 512]");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, SW2D_E_NODEV, "no HIP device");
+    if (params->device < 0 || params->device >= ndev) return fail(nullptr, SW2D_E_NODEV, "bad device ordinal");
+
+    sw2d_handle *h = new sw2d_handle();
+    h->grid = *grid;
+    h->prm = *params;
+    h->R = grid->rows; h->C = grid->cols;
+    h->device = params->device;
+    h->esize = params->precision == SW2D_FP64 ? 8 : 4;
+    int rc = SW2D_OK;
+    auto bail = [&](int code) {
+        g_create_error = h->err;
+        sw2d_destroy(h);
+        return code;
+    };
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) { hip_fail(h, e, "hipSetDevice"); return bail(SW2D_E_HIP); }
+    e = hipStreamCreate(&h->stream);  // blocking stream: ordered with hipMemcpy on the null stream
+    if (e != hipSuccess) { hip_fail(h, e, "hipStreamCreate"); return bail(SW2D_E_HIP); }
+
+    const int R = h->R, C = h->C;
+    const size_t RC = (size_t)R * C;
+    const double nodata = grid->nodata;
+
+    // host copy of z (outputs print it) and initial volume
+    double v0 = 0.0;
+    for (size_t i = 0; i < RC; i++)
+        if (z[i] != nodata) v0 += h0[i];
+    h->volume0 = v0 * grid->cellsize * grid->cellsize;
+
+    if (fused(h)) {
+        h->P = round_up(C, 16);
+        const size_t n = (size_t)R * h->P;
+        h->W = 58;
+        h->H = choose_tile_rows(h);
+        h->nstrips = (C + h->W - 1) / h->W;
+        h->nchunks = (R + h->H - 1) / h->H;
+        h->ntiles = h->nstrips * h->nchunks;
+        h->nparts = h->ntiles;
+        void **bufs[] = {&h->h[0], &h->h[1], &h->um[0], &h->um[1], &h->vn[0], &h->vn[1], &h->z};
+        for (void **b : bufs) {
+            if (h->esize == 8) { if ((rc = dalloc(h, (double **)b, n))) return bail(rc); }
+            else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
+        }
+        if ((rc = dalloc(h, &h->scratch, RC))) return bail(rc);
+        // pitched host images: inactive cells carry NaN in z
+        if (h->esize == 8) {
+            std::vector<double> zi(n, 0.0), hi(n, 0.0);
+            for (int r = 0; r < R; r++)
+                for (int j = 0; j < C; j++) {
+                    const double zv = z[(size_t)r * C + j];
+                    zi[(size_t)r * h->P + j] = (zv != nodata) ? zv : NAN;
+                    hi[(size_t)r * h->P + j] = h0[(size_t)r * C + j];
+                }
+            e = hipMemcpy(h->z, zi.data(), n * 8, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 8, hipMemcpyHostToDevice);
+        } else {
+            std::vector<float> zi(n, 0.0f), hi(n, 0.0f);
+            for (int r = 0; r < R; r++)
+                for (int j = 0; j < C; j++) {
+                    const double zv = z[(size_t)r * C + j];
+                    zi[(size_t)r * h->P + j] = (zv != nodata) ? (float)zv : NAN;
+                    hi[(size_t)r * h->P + j] = (float)h0[(size_t)r * C + j];
+                }
+            e = hipMemcpy(h->z, zi.data(), n * 4, hipMemcpyHostToDevice);
+            if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 4, hipMemcpyHostToDevice);
+        }
+        if (e != hipSuccess) { hip_fail(h, e, "upload"); return bail(SW2D_E_HIP); }
+    } else {
+        h->N = R * C + C / 2;
+        h->NA = h->N + 2 * C + 4;
+        double **arrs[] = {&h->pz, &h->ph, &h->pho, &h->phm, &h->phn, &h->pum, &h->pumo, &h->puu1,
+                           &h->pvv1, &h->puua, &h->pvva, &h->pvn, &h->pvno, &h->puu, &h->pvv};
+        for (double **a : arrs)
+            if ((rc = dalloc(h, a, h->NA))) return bail(rc);
+        if ((rc = dalloc(h, &h->pinf, h->NA))) return bail(rc);
+        std::vector<int> inf(h->NA, 0);
+        for (size_t i = 0; i < RC; i++) inf[i] = (z[i] != nodata) ? 1 : 0;
+        e = hipMemcpy(h->pz, z, RC * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->ph, h0, RC * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(h->pinf, inf.data(), h->NA * sizeof(int), hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hip_fail(h, e, "upload"); return bail(SW2D_E_HIP); }
+        int blocks = (h->N + 255) / 256;
+        h->plain_blocks = std::min(blocks, 2048);
+        h->nparts = h->plain_blocks;
+    }
+
+    if ((rc = dalloc(h, &h->clk, 3)) || (rc = dalloc(h, &h->part, 2 * (size_t)h->nparts)) ||
+        (rc = dalloc(h, &h->ledger, 1)) || (rc = dalloc(h, &h->peek, 1)))
+        return bail(rc);
+    Clock ck[3];
+    memset(ck, 0, sizeof(ck));
+    for (auto &c : ck) c.cfl_bits = kInfBits;
+    ck[0].t = params->time0;
+    e = hipMemcpy(h->clk, ck, sizeof(ck), hipMemcpyHostToDevice);
+    if (e != hipSuccess) { hip_fail(h, e, "clock upload"); return bail(SW2D_E_HIP); }
+    h->host_time = params->time0;
+    h->host_mstep = 0;
+    h->step_index = 0;
+
+    e = fused(h) ? launch_init(h) : launch_plain_init(h);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) { hip_fail(h, e, "init kernel"); return bail(SW2D_E_HIP); }
+    *out = h;
+    return SW2D_OK;
+}
+
+int sw2d_destroy(sw2d_handle *h)
+{
+    if (!h) return SW2D_OK;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    for (auto &ev : h->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    for (void *p : h->allocs) (void)hipFree(p);
+    h->allocs.clear();
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return SW2D_OK;
+}
+
+int sw2d_set_rain_series(sw2d_handle *h, const double *rain_mm, int n, double dtrain)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (n < 0 || (n > 0 && !rain_mm)) return fail(h, SW2D_E_ARG, "bad rain series");
+    if (!(dtrain > 0)) return fail(h, SW2D_E_ARG, "dtrain must be positive");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, h->rain);
+    h->rain = nullptr;
+    int rc = dalloc(h, &h->rain, n);
+    if (rc) return rc;
+    if (n > 0) HIPCHK(h, hipMemcpy(h->rain, rain_mm, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->nrain = n;
+    h->dtrain = dtrain;
+    return SW2D_OK;
+}
+
+int sw2d_set_sources(sw2d_handle *h, const double *x, const double *y, int nst, const double *Q,
+                     int nq, double dtoq)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (nst < 0 || nq < 0 || (nst > 0 && (!x || !y || (nq > 0 && !Q))))
+        return fail(h, SW2D_E_ARG, "bad source arguments");
+    if (nst > 0 && !(dtoq > 0)) return fail(h, SW2D_E_ARG, "dtoq must be positive");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    // cell index of stream_flow (SW2D-EFDC.cu:541-544)
+    const double res = h->grid.cellsize;
+    const double xcoor = h->grid.xllcorner / res, ycoor = h->grid.yllcorner / res;
+    std::vector<int> rr(nst), cc(nst);
+    for (int i = 0; i < nst; i++) {
+        const double brx = x[i] / res, bry = y[i] / res;
+        const int qinx = (int)round(fabs(xcoor - brx));
+        const int qiny = h->R - (int)round(fabs(ycoor - bry));
+        const long idx = (long)(qiny + 1) * h->C - (h->C - (qinx + 1));
+        if (idx < 0 || idx >= (long)h->R * h->C) {
+            char m[160];
+            snprintf(m, sizeof m, "source %d at (%g, %g) maps to cell %ld outside the grid", i, x[i], y[i], idx);
+            return fail(h, SW2D_E_ARG, m);
+        }
+        rr[i] = (int)(idx / h->C);
+        cc[i] = (int)(idx % h->C);
+    }
+    int rc = upload_source_cells(h, rr.data(), cc.data(), nst);
+    if (rc) return rc;
+    dfree(h, h->Q);
+    h->Q = nullptr;
+    if ((rc = dalloc(h, &h->Q, (size_t)nq * nst))) return rc;
+    if ((size_t)nq * nst > 0)
+        HIPCHK(h, hipMemcpy(h->Q, Q, sizeof(double) * nq * nst, hipMemcpyHostToDevice));
+    h->nst = nst; h->nq = nq; h->dtoq = dtoq;
+    return SW2D_OK;
+}
+
+int sw2d_set_evaporation(sw2d_handle *h, const double *evapo_mm, int n)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (n < 0 || (n > 0 && !evapo_mm)) return fail(h, SW2D_E_ARG, "bad evaporation series");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, h->evapo);
+    h->evapo = nullptr;
+    int rc = dalloc(h, &h->evapo, n);
+    if (rc) return rc;
+    if (n > 0) HIPCHK(h, hipMemcpy(h->evapo, evapo_mm, sizeof(double) * n, hipMemcpyHostToDevice));
+    h->nevapo = n;
+    h->evap_on = 1;
+    return SW2D_OK;
+}
+
+int sw2d_set_manning_map(sw2d_handle *h, const unsigned char *cls, const double *table, int nclass)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (!cls || !table || nclass < 1 || nclass > kMaxClasses)
+        return fail(h, SW2D_E_ARG, "bad Manning map");
+    const size_t RC = (size_t)h->R * h->C;
+    for (size_t i = 0; i < RC; i++)
+        if (cls[i] >= nclass) return fail(h, SW2D_E_ARG, "Manning class out of range");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (int i = 0; i < kMaxClasses; i++) h->ntab[i] = i < nclass ? table[i] : 0.0;
+    int rc;
+    if (fused(h)) {
+        if (!h->cls && (rc = dalloc(h, &h->cls, (size_t)h->R * h->P))) return rc;
+        std::vector<uint8_t> img((size_t)h->R * h->P, 0);
+        for (int r = 0; r < h->R; r++)
+            for (int j = 0; j < h->C; j++) img[(size_t)r * h->P + j] = cls[(size_t)r * h->C + j];
+        HIPCHK(h, hipMemcpy(h->cls, img.data(), img.size(), hipMemcpyHostToDevice));
+    } else {
+        if (!h->pncell && (rc = dalloc(h, &h->pncell, h->NA))) return rc;
+        std::vector<double> nc(h->NA, 0.0);
+        for (size_t i = 0; i < RC; i++) nc[i] = table[cls[i]];
+        HIPCHK(h, hipMemcpy(h->pncell, nc.data(), nc.size() * 8, hipMemcpyHostToDevice));
+    }
+    h->ncls_on = 1;
+    return SW2D_OK;
+}
+
+int sw2d_set_storm(sw2d_handle *h, const sw2d_storm *storm)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (!storm || !(storm->sigma > 0)) return fail(h, SW2D_E_ARG, "bad storm");
+    if (fused(h) && h->H > 512) return fail(h, SW2D_E_ARG, "tile too tall for the storm table");
+    h->storm = *storm;
+    h->storm_on = 1;
+    return SW2D_OK;
+}
+
+int sw2d_set_outlet(sw2d_handle *h, int row, int col)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (row < 0 || row >= h->R || col < 0 || col >= h->C) return fail(h, SW2D_E_ARG, "outlet outside the grid");
+    int edge = 0;
+    if (row == h->R - 1) edge = 1;
+    else if (row == 0) edge = 2;
+    else if (col == 0) edge = 3;
+    else if (col == h->C - 1) edge = 4;
+    if (!edge) return fail(h, SW2D_E_ARG, "outlet must be on a domain edge");
+    h->out_r = row; h->out_c = col; h->out_edge = edge;
+    return SW2D_OK;
+}
+
+int sw2d_set_hydrograph(sw2d_handle *h, long cap)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (cap < 0) return fail(h, SW2D_E_ARG, "negative capacity");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, h->hydro);
+    h->hydro = nullptr;
+    int rc = dalloc(h, &h->hydro, 2 * (size_t)cap);
+    if (rc) return rc;
+    h->hydro_cap = cap;
+    return SW2D_OK;
+}
+
+int sw2d_set_lake_bc(sw2d_handle *h, const int *mask)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (!mask) return fail(h, SW2D_E_ARG, "mask is NULL");
+    const int R = h->R, C = h->C;
+    const int cap = R * C;
+    std::vector<int> cells(4 * (size_t)cap), I(4 * (size_t)cap), J(4 * (size_t)cap), counts(4);
+    std::vector<double> q(4 * (size_t)cap);
+    std::vector<double> zeros((size_t)R * C, 0.0);
+    int rc = sw2d_couple_bc(R, C, h->grid.cellsize, h->grid.cellsize, mask, zeros.data(),
+                            zeros.data(), zeros.data(), cap, cells.data(), I.data(), J.data(),
+                            q.data(), counts.data());
+    if (rc) return fail(h, rc, "coupling scan failed");
+    std::vector<int> br, bc, bk;
+    for (int k = 0; k < 4; k++)
+        for (int i = 0; i < counts[k]; i++) {
+            const int p = cells[(size_t)k * cap + i];
+            br.push_back(p / C); bc.push_back(p % C); bk.push_back(k);
+        }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    dfree(h, h->bc_r); dfree(h, h->bc_c); dfree(h, h->bc_kind); dfree(h, h->bc_acc);
+    h->bc_r = h->bc_c = h->bc_kind = nullptr;
+    h->bc_acc = nullptr;
+    const int n = (int)br.size();
+    if ((rc = dalloc(h, &h->bc_r, n)) || (rc = dalloc(h, &h->bc_c, n)) ||
+        (rc = dalloc(h, &h->bc_kind, n)) || (rc = dalloc(h, &h->bc_acc, n)))
+        return rc;
+    if (n > 0) {
+        HIPCHK(h, hipMemcpy(h->bc_r, br.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->bc_c, bc.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        HIPCHK(h, hipMemcpy(h->bc_kind, bk.data(), n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->nbc = n;
+    return SW2D_OK;
+}
+
+int sw2d_step(sw2d_handle *h, int n_iters)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (n_iters < 0) return fail(h, SW2D_E_ARG, "negative iteration count");
+    for (int i = 0; i < n_iters; i++) {
+        const StepCommon sc = make_common(h);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (h->timing) {
+            HIPCHK(h, hipEventCreate(&e0));
+            HIPCHK(h, hipEventCreate(&e1));
+            HIPCHK(h, hipEventRecord(e0, h->stream));
+        }
+        hipError_t e = fused(h) ? launch_fused(h, sc) : launch_plain(h, sc);
+        if (e != hipSuccess) return hip_fail(h, e, "step launch");
+        if (h->timing) {
+            HIPCHK(h, hipEventRecord(e1, h->stream));
+            h->events.emplace_back(e0, e1);
+            if (h->events.size() >= 1024) {
+                HIPCHK(h, hipStreamSynchronize(h->stream));
+                for (auto &ev : h->events) {
+                    float ms = 0;
+                    HIPCHK(h, hipEventElapsedTime(&ms, ev.first, ev.second));
+                    h->kernel_ms += ms;
+                    (void)hipEventDestroy(ev.first);
+                    (void)hipEventDestroy(ev.second);
+                }
+                h->events.clear();
+            }
+            h->launches++;
+        }
+        if (fused(h)) h->cur = 1 - h->cur;
+        h->step_index++;
+        if (!h->prm.adaptive) {
+            h->host_time = h->host_time + h->prm.dt;
+            h->host_time = h->host_time + h->prm.dt;
+        }
+        h->host_mstep += 2;
+    }
+    return SW2D_OK;
+}
+
+int sw2d_sync(sw2d_handle *h)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return SW2D_OK;
+}
+
+int sw2d_time(sw2d_handle *h, double *time, long *mstep, double *dt2_next)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    Clock ck;
+    HIPCHK(h, hipMemcpy(&ck, h->clk + (h->step_index % 3), sizeof(Clock), hipMemcpyDeviceToHost));
+    if (time) *time = ck.t;
+    if (mstep) *mstep = (long)h->host_mstep;
+    if (dt2_next) {
+        if (h->prm.adaptive) {
+            const double m = __builtin_bit_cast(double, ck.cfl_bits);
+            const double d = h->prm.courant * m;
+            const double mx = 2.00 * h->prm.dt;
+            *dt2_next = (d < mx) ? d : mx;
+        } else {
+            *dt2_next = 2.00 * h->prm.dt;
+        }
+    }
+    return SW2D_OK;
+}
+
+int sw2d_run_until(sw2d_handle *h, double tmax, long max_iters, long *done)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    long n = 0;
+    if (!h->prm.adaptive) {
+        // the host mirror of the clock is exact in fixed-dt mode
+        double t = h->host_time;
+        long cnt = 0;
+        while (cnt < max_iters && t + h->prm.dt <= tmax) {
+            t = t + h->prm.dt;
+            t = t + h->prm.dt;
+            cnt++;
+        }
+        while (n < cnt) {
+            const int chunk = (int)std::min<long>(cnt - n, 1 << 20);
+            int rc = sw2d_step(h, chunk);
+            if (rc) return rc;
+            n += chunk;
+        }
+    } else {
+        const double dt2_max = 2.00 * h->prm.dt;
+        for (;;) {
+            if (n >= max_iters) break;
+            double t, dt2;
+            int rc = sw2d_time(h, &t, nullptr, &dt2);
+            if (rc) return rc;
+            if (!(t + 0.5 * dt2 <= tmax)) break;
+            long safe = (long)floor((tmax - t) / dt2_max);
+            if (safe < 1) safe = 1;
+            safe = std::min(safe, max_iters - n);
+            safe = std::min<long>(safe, 1 << 16);
+            if ((rc = sw2d_step(h, (int)safe))) return rc;
+            n += safe;
+        }
+    }
+    if (done) *done = n;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    return SW2D_OK;
+}
+
+int sw2d_get_field(sw2d_handle *h, int field, double *out)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (!out) return fail(h, SW2D_E_ARG, "out is NULL");
+    if (field < 0 || field > SW2D_FIELD_VV1) return fail(h, SW2D_E_ARG, "unknown field");
+    const size_t RC = (size_t)h->R * h->C;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (fused(h)) {
+        hipError_t e = launch_derive(h, field, h->scratch);
+        if (e != hipSuccess) return hip_fail(h, e, "derive");
+        HIPCHK(h, hipMemcpyAsync(out, h->scratch, RC * 8, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if (field == SW2D_FIELD_Z)
+            for (size_t i = 0; i < RC; i++)
+                if (out[i] != out[i]) out[i] = h->grid.nodata;
+        return SW2D_OK;
+    }
+    const double *src = nullptr;
+    switch (field) {
+    case SW2D_FIELD_H: src = h->ph; break;
+    case SW2D_FIELD_UM: src = h->pum; break;
+    case SW2D_FIELD_VN: src = h->pvn; break;
+    case SW2D_FIELD_UU: src = h->puu; break;
+    case SW2D_FIELD_VV: src = h->pvv; break;
+    case SW2D_FIELD_UUA: src = h->puua; break;
+    case SW2D_FIELD_VVA: src = h->pvva; break;
+    case SW2D_FIELD_Z: src = h->pz; break;
+    case SW2D_FIELD_HM: src = h->phm; break;
+    case SW2D_FIELD_HN: src = h->phn; break;
+    case SW2D_FIELD_UU1: src = h->puu1; break;
+    case SW2D_FIELD_VV1: src = h->pvv1; break;
+    }
+    HIPCHK(h, hipMemcpy(out, src, RC * 8, hipMemcpyDeviceToHost));
+    return SW2D_OK;
+}
+
+int sw2d_balance(sw2d_handle *h, sw2d_balance_out *out)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (!out) return fail(h, SW2D_E_ARG, "out is NULL");
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const StepCommon sc = make_common(h);
+    hipError_t e = launch_peek(h, sc);
+    if (e != hipSuccess) return hip_fail(h, e, "ledger peek");
+    Ledger L;
+    HIPCHK(h, hipMemcpyAsync(&L, h->peek, sizeof(L), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    out->volume = (h->step_index > 0) ? L.vol_last : h->volume0;
+    out->volume0 = h->volume0;
+    out->v_source = L.v_src;
+    out->v_clip = L.v_clip;
+    out->v_outflow = L.v_out;
+    out->q_out = L.q_last;
+    out->q_peak = L.q_peak;
+    out->t_peak = L.t_peak;
+    out->error = out->volume - (out->volume0 + out->v_source + out->v_clip - out->v_outflow);
+    return SW2D_OK;
+}
+
+int sw2d_hydrograph(sw2d_handle *h, double *out, long cap, long *n)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    sw2d_balance_out b;
+    int rc = sw2d_balance(h, &b);  // writes the pending sample of the last iteration
+    if (rc) return rc;
+    long have = (long)std::min<long long>(h->step_index, h->hydro_cap);
+    long m = std::min(have, cap);
+    if (m > 0 && out) HIPCHK(h, hipMemcpy(out, h->hydro, sizeof(double) * 2 * m, hipMemcpyDeviceToHost));
+    if (n) *n = have;
+    return SW2D_OK;
+}
+
+int sw2d_lake_inflow(sw2d_handle *h, double *out, int cap, int *n)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int m = std::min(cap, h->nbc);
+    if (m > 0 && out) HIPCHK(h, hipMemcpy(out, h->bc_acc, sizeof(double) * m, hipMemcpyDeviceToHost));
+    if (n) *n = h->nbc;
+    return SW2D_OK;
+}
+
+int sw2d_timing(sw2d_handle *h, int enable)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (auto &ev : h->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    h->events.clear();
+    h->kernel_ms = 0.0;
+    h->launches = 0;
+    h->timing = enable ? 1 : 0;
+    return SW2D_OK;
+}
+
+int sw2d_kernel_time(sw2d_handle *h, double *ms, long *launches)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (auto &ev : h->events) {
+        float m = 0;
+        HIPCHK(h, hipEventElapsedTime(&m, ev.first, ev.second));
+        h->kernel_ms += m;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    h->events.clear();
+    if (ms) *ms = h->kernel_ms;
+    if (launches) *launches = h->launches;
+    return SW2D_OK;
+}
+
+int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
+{
+    if (check_idle(h) || !bytes) return SW2D_E_ARG;
+    long long b = 0;
+    const size_t RC = (size_t)h->R * h->C;
+    if (fused(h)) b += (long long)7 * h->R * h->P * h->esize + (long long)RC * 8 + (h->cls ? (long long)h->R * h->P : 0);
+    else b += (long long)16 * h->NA * 8;
+    b += (long long)2 * h->nparts * sizeof(Partial);
+    *bytes = b;
+    return SW2D_OK;
+}
+
+}  // extern "C"
+
+// ---- device math probe ----
+namespace {
+__global__ void math_probe(int fn, const double *x, const double *y, double *out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i];
+    const double b = y ? y[i] : 0.0;
+    double r = 0;
+    switch (fn) {
+    case 0: r = pow1333(a); break;
+    case 1: r = exp_d(a); break;
+    case 2: r = log_d(a); break;
+    case 3: r = __builtin_sqrt(a); break;
+    case 4: r = a / b; break;
+    case 5: r = __builtin_fmax(a, b); break;
+    default: r = 0; break;
+    }
+    out[i] = r;
+}
+}  // namespace
+
+extern "C" int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n)
+{
+    if (!x || !out || n < 0 || fn < 0 || fn > 5) return SW2D_E_ARG;
+    if ((fn == 4 || fn == 5) && !y) return SW2D_E_ARG;
+    if (n == 0) return SW2D_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SW2D_E_NODEV;
+    if (hipSetDevice(device) != hipSuccess) return SW2D_E_HIP;
+    double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+    const size_t bytes = sizeof(double) * n;
+    int rc = SW2D_OK;
+    if (hipMalloc(&dx, bytes) != hipSuccess || hipMalloc(&dout, bytes) != hipSuccess ||
+        (y && hipMalloc(&dy, bytes) != hipSuccess))
+        rc = SW2D_E_NOMEM;
+    if (!rc && hipMemcpy(dx, x, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SW2D_E_HIP;
+    if (!rc && y && hipMemcpy(dy, y, bytes, hipMemcpyHostToDevice) != hipSuccess) rc = SW2D_E_HIP;
+    if (!rc) {
+        hipLaunchKernelGGL(math_probe, dim3((n + 255) / 256), dim3(256), 0, 0, fn, dx, dy, dout, n);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = SW2D_E_HIP;
+    }
+    if (!rc && hipMemcpy(out, dout, bytes, hipMemcpyDeviceToHost) != hipSuccess) rc = SW2D_E_HIP;
+    if (dx) (void)hipFree(dx);
+    if (dy) (void)hipFree(dy);
+    if (dout) (void)hipFree(dout);
+    return rc;
+}

@@ -1,0 +1,526 @@
+// sw2d_fused.hip -- one fused kernel per SW2D loop iteration.
+//
+// The reference (SW2D-EFDC.cu:4798-4877) runs nine grid-stride kernels per
+// iteration over ~20 FP64 arrays: flux, continuity, treat_error, continuity,
+// hm_hn, uu1_vv1, uu_vv, uua_vva, forward.  The complete state between
+// iterations is only (h, um, vn) (bed z is constant); every other array is a
+// local function of it.  This kernel reads (h, um, vn, z) once and writes
+// (h, um, vn) once per iteration, so one iteration moves 7 arrays:
+// 56 B/cell in FP64, 28 B/cell in FP32.
+//
+// Work decomposition.  A tile is one wavefront (64 lanes) and covers 58
+// output columns x H output rows.  Lane l holds column j = j0 - 3 + l: the
+// dependency chain flux(+-1) -> continuity (+-1) -> treat_error (+-1) ->
+// continuity needs 3 cells of halo on each side.  The wave streams the rows
+// of its tile from north to south; when row k arrives it computes
+//   the face depths/velocities of row k            (hm_hn, uu1_vv1, uua_vva)
+//   the fluxes of row k-1                          (flux)
+//   the first continuity of row k-2                (continuity at t)
+//   treat_error on the faces of row k-2
+//   the second continuity of row k-3              (continuity at t + dt)
+// and stores row k-3 of the new state.  Values of a row are carried in
+// registers while they are needed (7-row window); east/west neighbours are
+// cross-lane shuffles.  No LDS staging and no barriers: each wave is
+// independent.  The grid is one wave per tile plus one bookkeeping wave.
+//
+// Every FP64 expression is written in the reference's evaluation order and
+// the file is compiled with -ffp-contract=off, so the FP64 results equal the
+// reference's arithmetic (as restated by the CPU FP64 check) bit for bit.
+#include "sw2d_common.hpp"
+
+namespace sw2d {
+
+template <typename T> __device__ __forceinline__ T from_east(T v) { return __shfl_down(v, 1, 64); }
+template <typename T> __device__ __forceinline__ T from_west(T v) { return __shfl_up(v, 1, 64); }
+
+// bit of lane-1 of a wave-wide predicate
+__device__ __forceinline__ bool west_bit(bool p, int lane)
+{
+    unsigned long long m = __ballot(p);
+    return lane > 0 && ((m >> (lane - 1)) & 1ULL);
+}
+
+constexpr int kMaxTileRows = 512;
+
+template <typename T, bool ADAPT>
+__global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
+{
+    using M = num<T>;
+    const StepCommon &s = a.s;
+    const int lane = threadIdx.x;
+    const StepTime st = step_time(s);
+
+    if ((int)blockIdx.x == a.ntiles) {
+        PitchedState<T> ps{a.h_in, a.um_in, a.vn_in, s.R, s.C, a.P, (T)s.th};
+        auto vel = [&](int r, int j, double &hh, double &ua, double &va) {
+            hh = (double)ps.hv(r, j);
+            ua = (double)ps.uua(r, j);
+            va = (double)ps.vva(r, j);
+        };
+        bookkeeping(s, st, vel, lane);
+        return;
+    }
+
+    // tile of this wave; with xcd_remap consecutive tiles go to one XCD
+    int tile = blockIdx.x;
+    if (a.xcd_remap) {
+        const int nx = 8, per = a.ntiles / nx, rem = a.ntiles % nx;
+        const int x = tile % nx, i = tile / nx;
+        tile = x * per + (x < rem ? x : rem) + i;
+    }
+    const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
+    const int R = s.R, C = s.C, P = a.P;
+    const int j0 = strip * a.W;
+    const int j = j0 - 3 + lane;
+    const bool colin = (j >= 0) && (j < C);
+    const bool colout = (lane >= 3) && (lane < 3 + a.W) && (j < C);
+    const int r0 = chunk * a.H;
+    const int r1 = (r0 + a.H < R) ? r0 + a.H : R;
+
+    const T th = (T)s.th, gg = (T)s.g, dx = (T)s.dx, dy = (T)s.dy;
+    const T dt2 = (T)st.dt2;
+    const T dt2dx = (T)(st.dt2 / s.dx), dt2dy = (T)(st.dt2 / s.dy);
+    const T rn = (T)s.rn;
+    const T hdt2 = (T)st.dt2;  // dt2 of the friction factor (dt2*ram*0.50)
+
+    // forcing of this iteration: first continuity at t, second at t + dt
+    const double rr1 = rain_rate(s, st.t), rr2 = rain_rate(s, st.tm);
+    const double ev1 = evap_rate(s, st.t), ev2 = evap_rate(s, st.tm);
+    const bool storm1 = storm_live(s, st.t), storm2 = storm_live(s, st.tm);
+    double gx1 = 0, gx2 = 0;
+    __shared__ double gy_s[2][kMaxTileRows + 8];
+    if (s.storm_on) {
+        const double x = ((double)j + 0.5) * s.dx;
+        gx1 = storm_factor(x - (s.st_x0 + s.st_vx * st.t), s.st_sigma);
+        gx2 = storm_factor(x - (s.st_x0 + s.st_vx * st.tm), s.st_sigma);
+        for (int i = lane; i < a.H + 6; i += 64) {
+            const double y1 = ((double)(r0 - 5 + i) + 0.5) * s.dy;
+            const double y2 = ((double)(r0 - 6 + i) + 0.5) * s.dy;
+            gy_s[0][i] = storm_factor(y1 - (s.st_y0 + s.st_vy * st.t), s.st_sigma);
+            gy_s[1][i] = storm_factor(y2 - (s.st_y0 + s.st_vy * st.tm), s.st_sigma);
+        }
+        __syncthreads();
+    }
+    const T P_lake = (T)s.P_lake, P_land = (T)s.P_land, ewl = (T)s.EV_WL_min;
+
+    // outlet
+    const bool is_out_col = s.out_edge && (j == s.out_c);
+    T qo_eff = 0;
+
+    // carried rows: suffix 1 = row k-1, 2 = k-2, 3 = k-3
+    T h1 = 0, h2 = 0, h3 = 0, h1w = 0, z1 = 0, z2 = 0, z1w = 0;
+    bool a1 = false, a2 = false, a3 = false, a1w = false;
+    T u1 = 0, u2 = 0, u1e = 0, u1w = 0, v1 = 0, v2 = 0, v1e = 0, v1w = 0;
+    T uu1_1 = 0, uu1_2 = 0, uu1_1w = 0;
+    T vv1_1 = 0, vv1_2 = 0, vv1_1e = 0, vv1_2e = 0, vv1_1w = 0;
+    T hm1 = 0, hn1 = 0, vva1 = 0;
+    T nc1 = rn, nc2 = rn, nc1w = rn;
+    T umnP = 0, vnnP = 0, umpP = 0, vnpP = 0;
+    bool dryP = false;
+
+    double acc_vol = 0, acc_src = 0, acc_clip = 0, acc_q = 0, cfl = INFINITY;
+
+    // software pipeline: row k+1 is loaded while row k is processed
+    const int kbeg = r0 - 3, kend = r1 + 3;
+    T nh = 0, nu = 0, nv = 0, nz = 0;
+    uint8_t ncl = 0;
+    {
+        const int k = kbeg;
+        if (k >= 0 && k < R && colin) {
+            const long off = (long)k * P + j;
+            nh = a.h_in[off]; nu = a.um_in[off]; nv = a.vn_in[off]; nz = a.z[off];
+            if (s.ncls_on) ncl = a.cls[off];
+        } else {
+            nz = __builtin_nan("");
+        }
+    }
+
+    for (int k = kbeg; k < kend; ++k) {
+        const T h0 = nh, u0 = nu, v0 = nv, z0 = nz;
+        const uint8_t cl0 = ncl;
+        if (k + 1 < kend) {
+            const int kn = k + 1;
+            if (kn >= 0 && kn < R && colin) {
+                const long off = (long)kn * P + j;
+                nh = a.h_in[off]; nu = a.um_in[off]; nv = a.vn_in[off]; nz = a.z[off];
+                if (s.ncls_on) ncl = a.cls[off];
+            } else {
+                nh = 0; nu = 0; nv = 0; nz = __builtin_nan(""); ncl = 0;
+            }
+        }
+        const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
+        const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
+
+        // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
+        const T h0w = from_west(h0);
+        const T z0w = from_west(z0);
+        const bool a0w = west_bit(a0, lane);
+        const T u0e = from_east(u0), u0w = from_west(u0);
+        const T v0e = from_east(v0), v0w = from_west(v0);
+        const T nc0w = from_west(nc0);
+        const T hn0 = (j == 0) ? h0 : T(0.50) * (h0 + h0w);
+        const T hm0 = (k == 0) ? h0 : ((k == R) ? h1 : T(0.50) * (h0 + h1));
+        const T uu1_0 = (hm0 >= th) ? u0 / hm0 : T(0.00);
+        const T vv1_0 = (hn0 >= th) ? v0 / hn0 : T(0.0);
+        const T vv1_0e = from_east(vv1_0), vv1_0w = from_west(vv1_0);
+        const T uu1_0w = from_west(uu1_0);
+        // uua_vva (489-522): vva of row k, uua of row k-1
+        const T vva0 = (k > 0) ? T(0.250) * (vv1_0 + vv1_0e + vv1_1 + vv1_1e)
+                               : T(0.50) * (vv1_0 + vv1_0e);
+        const T uua1 = (j > 0) ? T(0.250) * (uu1_1 + uu1_0 + uu1_1w + uu1_0w)
+                               : T(0.50) * (uu1_1 + uu1_0);
+
+        if (ADAPT && colout && k >= r0 && k < r1) {
+            // CFL candidates of the incoming state on the faces of row k
+            if (k >= 1 && a0 && a1 && hm0 > th) {
+                const double w = s.dx / ((double)M::abs_(uu1_0) + sqrt(s.g * (double)hm0));
+                cfl = (w < cfl) ? w : cfl;
+            }
+            if (j >= 1 && a0 && a0w && hn0 > th) {
+                const double w = s.dy / ((double)M::abs_(vv1_0) + sqrt(s.g * (double)hn0));
+                cfl = (w < cfl) ? w : cfl;
+            }
+        }
+
+        // ---- fluxes of row f = k-1 (flux, SW2D-EFDC.cu:97-353) ----
+        const int f = k - 1;
+        T umn = 0, vnn = 0;
+        // x direction: um at the north face of (f, j); w = row f-1, e = row f
+        if (f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th))) {
+            const T hhe = h1 + z1, hhw = h2 + z2, hhep = h1 - th, hhwp = h2 - th;
+            if (hhe < z2) {
+                umn = (h2 > th) ? T(0.5440) * h2 * M::sqrt_(gg * h2) : T(0);
+            } else if (hhw < z1) {
+                umn = (h1 > th) ? T(-0.544) * h1 * M::sqrt_(gg * h1) : T(0);
+            } else if (hhep * hhwp < 0) {
+                if ((h1 > 0) || (h2 > 0)) {
+                    const T hhan = hhep - hhwp;
+                    const T sgnm = hhan / M::abs_(hhan);
+                    const T hh3 = M::max_((h1 + z1), (h2 + z2)) - M::max_(z1, z2);
+                    umn = -sgnm * T(0.350) * hh3 * M::sqrt_(T(2.00) * gg * hh3);
+                } else {
+                    umn = 0;
+                }
+            } else {
+                const T u13 = gg * hm1 * dt2dx * (h1 + z1 - h2 - z2);
+                const T ur = T(0.50) * (uu1_0 + uu1_1);
+                const T ul = T(0.50) * (uu1_1 + uu1_2);
+                const T umr = ur * (u0 + u1) * T(0.50) + M::abs_(ur) * (u1 - u0) * T(0.50);
+                const T uml = ul * (u1 + u2) * T(0.50) + M::abs_(ul) * (u2 - u1) * T(0.50);
+                const T u11 = dt2dx * (umr - uml);
+                T u12 = 0;
+                if (j != 0) {
+                    const T vu = T(0.50) * (vv1_1e + vv1_2e);
+                    const T vd = T(0.50) * (vv1_1 + vv1_2);
+                    const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
+                    const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
+                    u12 = dt2dy * (umu - umd);
+                }
+                const T nf = s.ncls_on ? T(0.5) * (nc2 + nc1) : rn;
+                T ram = 0.00;
+                if (!(hm1 <= th)) {
+                    const T sqx = M::sqrt_(uu1_1 * uu1_1 + vva1 * vva1);
+                    ram = gg * nf * nf * sqx / pow1333(hm1);
+                }
+                umn = ((T(1.00) - hdt2 * ram * T(0.50)) * u1 + (-u11 - u12 - u13)) /
+                      (T(1.00) + hdt2 * ram * T(0.50));
+            }
+        }
+        // y direction: vn at the west face of (f, j); s = column j-1, n = column j
+        if (j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th))) {
+            const T hhn = h1 + z1, hhs = h1w + z1w, hhnp = h1 - th, hhsp = h1w - th;
+            if (hhn < z1w) {
+                vnn = (h1w > th) ? T(0.5440) * h1w * M::sqrt_(gg * h1w) : T(0);
+            } else if (hhs < z1) {
+                vnn = (h1 > th) ? T(-0.544) * h1 * M::sqrt_(gg * h1) : T(0);
+            } else if (hhnp * hhsp < 0) {
+                if ((h1 > 0) || (h1w > 0)) {
+                    const T hhan = hhnp - hhsp;
+                    const T sgnm = hhan / M::abs_(hhan);
+                    const T hh3 = M::max_((h1 + z1), (h1w + z1w)) - M::max_(z1, z1w);
+                    vnn = -sgnm * T(0.350) * hh3 * M::sqrt_(T(2.00) * gg * hh3);
+                } else {
+                    vnn = 0;
+                }
+            } else {
+                const T v13 = gg * hn1 * dt2dy * (h1 + z1 - h1w - z1w);
+                T v11 = 0;
+                if (f != 0 && f != R) {
+                    const T av = T(0.50) * (uu1_0 + uu1_0w);
+                    const T bv = T(0.50) * (uu1_1 + uu1_1w);
+                    const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
+                    const T vnl = bv * (v1 + v2) * T(0.50) + M::abs_(bv) * (v2 - v1) * T(0.50);
+                    v11 = dt2dx * (vnr - vnl);
+                }
+                const T pv = T(0.50) * (vv1_1e + vv1_1);
+                const T qv = T(0.50) * (vv1_1 + vv1_1w);
+                const T vnu = pv * (v1e + v1) * T(0.50) + M::abs_(pv) * (v1 - v1e) * T(0.50);
+                const T vnd = qv * (v1w + v1) * T(0.50) + M::abs_(qv) * (v1w - v1) * T(0.50);
+                const T v12 = dt2dy * (vnu - vnd);
+                const T nf = s.ncls_on ? T(0.5) * (nc1w + nc1) : rn;
+                T ram = 0.00;
+                if (!(hn1 <= th)) {
+                    const T sqy = M::sqrt_(uua1 * uua1 + vv1_1 * vv1_1);
+                    ram = gg * nf * nf * sqy / pow1333(hn1);
+                }
+                vnn = ((T(1.00) - hdt2 * ram * T(0.50)) * v1 + (-v11 - v12 - v13)) /
+                      (T(1.00) + hdt2 * ram * T(0.50));
+            }
+        }
+
+        // ---- first continuity of row g = k-2 at time t (602-664) ----
+        const int g = k - 2;
+        const T vnnPe = from_east(vnnP);
+        T h1v;
+        bool dryg;
+        {
+            const bool wet = h2 > ewl;
+            const T Pc = wet ? P_lake : P_land;
+            const T E = wet ? (T)ev1 : T(0.000);
+            T rr = (T)rr1;
+            if (s.storm_on) {
+                const int gi = g - (r0 - 5);
+                rr = storm1 ? (T)(s.st_peak * (gx1 * gy_s[0][gi < 0 ? 0 : gi])) : T(0);
+            }
+            T ql = 0;
+            for (int i = 0; i < s.nst; ++i)
+                if (s.src_r[i] == g && s.src_c[i] == j) ql = (T)source_rate(s, i, st.t);
+            T ums = umn, umnn = umnP, vne = vnnPe, vnw = vnnP;
+            if (is_out_col && g == s.out_r) {
+                const T qo = (h2 > th) ? T(0.5440) * h2 * M::sqrt_(gg * h2) : T(0);
+                if (s.out_edge == 1) ums = qo;
+                else if (s.out_edge == 2) umnn = -qo;
+                else if (s.out_edge == 3) vnw = -qo;
+                else vne = qo;
+                qo_eff = qo;
+            }
+            const T hr = h2 - dt2 * ((ums - umnn) / dx + (vne - vnw) / dy - ql - rr * Pc + E);
+            h1v = M::max_(hr, T(0.00));
+            if (!a2) h1v = 0.00;
+            dryg = a2 && (h1v < th);
+        }
+        // ---- treat_error on the faces of row g (695-725) ----
+        T umpg = umnP;
+        if (dryP && umpg > 0) umpg = 0;
+        if (dryg && umpg < 0) umpg = 0;
+        const bool drygw = west_bit(dryg, lane);
+        T vnpg = vnnP;
+        if (drygw && vnpg > 0) vnpg = 0;
+        if (dryg && vnpg < 0) vnpg = 0;
+        if (is_out_col && g == s.out_r && dryg) qo_eff = 0;
+
+        // ---- second continuity of row q = k-3 at time t + dt; store ----
+        const int q = k - 3;
+        const T vnpPe = from_east(vnpP);
+        {
+            const bool wet = h3 > ewl;
+            const T Pc = wet ? P_lake : P_land;
+            const T E = wet ? (T)ev2 : T(0.000);
+            T rr = (T)rr2;
+            if (s.storm_on) {
+                const int qi = q - (r0 - 6);
+                rr = storm2 ? (T)(s.st_peak * (gx2 * gy_s[1][qi < 0 ? 0 : qi])) : T(0);
+            }
+            T ql = 0;
+            for (int i = 0; i < s.nst; ++i)
+                if (s.src_r[i] == q && s.src_c[i] == j) ql = (T)source_rate(s, i, st.tm);
+            T ums = umpg, umnn = umpP, vne = vnpPe, vnw = vnpP;
+            const bool outcell = is_out_col && q == s.out_r;
+            if (outcell) {
+                if (s.out_edge == 1) ums = qo_eff;
+                else if (s.out_edge == 2) umnn = -qo_eff;
+                else if (s.out_edge == 3) vnw = -qo_eff;
+                else vne = qo_eff;
+            }
+            const T hr = h3 - dt2 * ((ums - umnn) / dx + (vne - vnw) / dy - ql - rr * Pc + E);
+            T h2v = M::max_(hr, T(0.00));
+            if (!a3) h2v = 0.00;
+            if (colout && q >= r0 && q < r1) {
+                const long off = (long)q * P + j;
+                a.h_out[off] = h2v;
+                a.um_out[off] = umpP;
+                a.vn_out[off] = vnpP;
+                if (a3) {
+                    acc_vol += (double)h2v;
+                    acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
+                    if (hr < 0) acc_clip += -(double)hr;
+                }
+                if (outcell) acc_q += (double)qo_eff * s.dy;
+            }
+        }
+
+        // ---- rotate the row window ----
+        h3 = h2; h2 = h1; h1 = h0; h1w = h0w;
+        z2 = z1; z1 = z0; z1w = z0w;
+        a3 = a2; a2 = a1; a1 = a0; a1w = a0w;
+        u2 = u1; u1 = u0; u1e = u0e; u1w = u0w;
+        v2 = v1; v1 = v0; v1e = v0e; v1w = v0w;
+        uu1_2 = uu1_1; uu1_1 = uu1_0; uu1_1w = uu1_0w;
+        vv1_2 = vv1_1; vv1_2e = vv1_1e; vv1_1 = vv1_0; vv1_1e = vv1_0e; vv1_1w = vv1_0w;
+        hm1 = hm0; hn1 = hn0; vva1 = vva0;
+        nc2 = nc1; nc1 = nc0; nc1w = nc0w;
+        umnP = umn; vnnP = vnn;
+        dryP = dryg; umpP = umpg; vnpP = vnpg;
+    }
+
+    // ---- per-tile partial sums (fixed-order wave reductions) ----
+    acc_vol = wave_sum(acc_vol);
+    acc_src = wave_sum(acc_src);
+    acc_clip = wave_sum(acc_clip);
+    acc_q = wave_sum(acc_q);
+    if (ADAPT) cfl = wave_min(cfl);
+    if (lane == 0) {
+        Partial p;
+        p.vol = acc_vol; p.src = acc_src; p.clip = acc_clip; p.qout = acc_q;
+        s.part_out[tile] = p;
+        if (ADAPT && cfl < INFINITY)
+            atomicMin(&s.clk[s.slot_b].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
+    }
+}
+
+// initial CFL minimum of the state (adaptive mode), one wave per tile rows
+template <typename T>
+__global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T *vn, const T *z,
+                                                int R, int C, int P, double dx, double dy,
+                                                double g, double th, Clock *clk)
+{
+    using M = num<T>;
+    const long n = (long)R * C;
+    double cfl = INFINITY;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(idx / C), j = (int)(idx % C);
+        const long o = (long)r * P + j;
+        const bool ac = (z[o] == z[o]);
+        if (r >= 1) {
+            const bool an = (z[o - P] == z[o - P]);
+            const T hm = T(0.50) * (h[o] + h[o - P]);
+            if (ac && an && hm > (T)th) {
+                const T uu1 = (hm >= (T)th) ? um[o] / hm : T(0.00);
+                const double w = dx / ((double)M::abs_(uu1) + sqrt(g * (double)hm));
+                cfl = (w < cfl) ? w : cfl;
+            }
+        }
+        if (j >= 1) {
+            const bool aw = (z[o - 1] == z[o - 1]);
+            const T hn = T(0.50) * (h[o] + h[o - 1]);
+            if (ac && aw && hn > (T)th) {
+                const T vv1 = (hn >= (T)th) ? vn[o] / hn : T(0.0);
+                const double w = dy / ((double)M::abs_(vv1) + sqrt(g * (double)hn));
+                cfl = (w < cfl) ? w : cfl;
+            }
+        }
+    }
+    cfl = wave_min(cfl);
+    if ((threadIdx.x & 63) == 0 && cfl < INFINITY)
+        atomicMin(&clk[0].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
+}
+
+// one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva)
+template <typename T>
+__global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
+                                                    const T *z, int R, int C, int P, double th,
+                                                    int field, double *out)
+{
+    PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
+    const long n = (long)R * C;
+    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(idx / C), j = (int)(idx % C);
+        const long o = (long)r * P + j;
+        T v = 0;
+        switch (field) {
+        case SW2D_FIELD_H: v = h[o]; break;
+        case SW2D_FIELD_UM: v = um[o]; break;
+        case SW2D_FIELD_VN: v = vn[o]; break;
+        case SW2D_FIELD_HM: v = (r == 0) ? h[o] : T(0.50) * (h[o] + h[o - P]); break;
+        case SW2D_FIELD_HN: v = (j == 0) ? h[o] : T(0.50) * (h[o] + h[o - 1]); break;
+        case SW2D_FIELD_UU1: v = ps.uu1(r, j); break;
+        case SW2D_FIELD_VV1: v = ps.vv1(r, j); break;
+        case SW2D_FIELD_UU: v = (h[o] >= (T)th) ? (ps.uu1(r + 1, j) + ps.uu1(r, j)) / T(2.00) : T(0.00); break;
+        case SW2D_FIELD_VV: v = (h[o] >= (T)th) ? (ps.vv1(r, j + 1) + ps.vv1(r, j)) / T(2.00) : T(0.00); break;
+        case SW2D_FIELD_UUA: v = ps.uua(r, j); break;
+        case SW2D_FIELD_VVA: v = ps.vva(r, j); break;
+        case SW2D_FIELD_Z: v = z[o]; break;
+        default: break;
+        }
+        out[idx] = (double)v;
+    }
+}
+
+__global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *out)
+{
+    ledger_peek(s, out, threadIdx.x);
+}
+
+template <typename T>
+static hipError_t launch_fused_t(sw2d_handle *h, const StepCommon &sc)
+{
+    FusedArgs<T> a;
+    const int in = h->cur, outb = 1 - h->cur;
+    a.h_in = (const T *)h->h[in]; a.um_in = (const T *)h->um[in]; a.vn_in = (const T *)h->vn[in];
+    a.h_out = (T *)h->h[outb]; a.um_out = (T *)h->um[outb]; a.vn_out = (T *)h->vn[outb];
+    a.z = (const T *)h->z;
+    a.cls = h->cls;
+    a.P = h->P; a.W = h->W; a.H = h->H;
+    a.nstrips = h->nstrips; a.ntiles = h->ntiles;
+    a.xcd_remap = 1;
+    a.s = sc;
+    dim3 grid(h->ntiles + 1), block(64);
+    if (h->prm.adaptive)
+        hipLaunchKernelGGL((fused_step<T, true>), grid, block, 0, h->stream, a);
+    else
+        hipLaunchKernelGGL((fused_step<T, false>), grid, block, 0, h->stream, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused(sw2d_handle *h, const StepCommon &sc)
+{
+    if (h->esize == 8) return launch_fused_t<double>(h, sc);
+    return launch_fused_t<float>(h, sc);
+}
+
+hipError_t launch_init(sw2d_handle *h)
+{
+    if (!h->prm.adaptive) return hipSuccess;
+    const long n = (long)h->R * h->C;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    if (h->esize == 8)
+        hipLaunchKernelGGL(init_cfl<double>, dim3(blocks), dim3(256), 0, h->stream,
+                           (const double *)h->h[h->cur], (const double *)h->um[h->cur],
+                           (const double *)h->vn[h->cur], (const double *)h->z, h->R, h->C, h->P,
+                           h->grid.cellsize, h->grid.cellsize, h->prm.g, h->prm.th, h->clk);
+    else
+        hipLaunchKernelGGL(init_cfl<float>, dim3(blocks), dim3(256), 0, h->stream,
+                           (const float *)h->h[h->cur], (const float *)h->um[h->cur],
+                           (const float *)h->vn[h->cur], (const float *)h->z, h->R, h->C, h->P,
+                           h->grid.cellsize, h->grid.cellsize, h->prm.g, h->prm.th, h->clk);
+    return hipGetLastError();
+}
+
+hipError_t launch_derive(sw2d_handle *h, int field, double *out_dev)
+{
+    const long n = (long)h->R * h->C;
+    int blocks = (int)((n + 255) / 256);
+    if (blocks > 8192) blocks = 8192;
+    if (h->esize == 8)
+        hipLaunchKernelGGL(derive_field<double>, dim3(blocks), dim3(256), 0, h->stream,
+                           (const double *)h->h[h->cur], (const double *)h->um[h->cur],
+                           (const double *)h->vn[h->cur], (const double *)h->z, h->R, h->C, h->P,
+                           h->prm.th, field, out_dev);
+    else
+        hipLaunchKernelGGL(derive_field<float>, dim3(blocks), dim3(256), 0, h->stream,
+                           (const float *)h->h[h->cur], (const float *)h->um[h->cur],
+                           (const float *)h->vn[h->cur], (const float *)h->z, h->R, h->C, h->P,
+                           h->prm.th, field, out_dev);
+    return hipGetLastError();
+}
+
+hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc)
+{
+    hipLaunchKernelGGL(peek_kernel, dim3(1), dim3(64), 0, h->stream, sc, h->peek);
+    return hipGetLastError();
+}
+
+}  // namespace sw2d

@@ -1,0 +1,145 @@
+// sw2d_math.hpp -- device math of the SW2D kernels.
+//
+// FP64: pow(x, 1.3333330) (the Manning depth exponent of SW2D-EFDC.cu:194,
+// 311) and exp() are built from IEEE-exact operations only (+ - * /, floor,
+// exponent bit manipulation), so that the result is a fixed function of the
+// input on every machine: the CPU FP64 check of this library evaluates the
+// same sequence and must agree bit for bit.  Translation units that include
+// this header are compiled with -ffp-contract=off.
+//
+// Method: x = 2^e m, m in (sqrt(1/2), sqrt(2)]; log m = 2 atanh(s),
+// s = (m-1)/(m+1), arranged as f - (hf - s (hf + R)), hf = f^2/2,
+// R(z) = sum_k 2 z^k / (2k+1), z = s^2 (the atanh-series arrangement of Sun's
+// fdlibm e_log.c with plain Taylor coefficients); exp(t) = 2^k exp(r),
+// r = t - k ln2 with a two-constant (Cody-Waite) ln2, exp(r) by Taylor to
+// r^13.  Error against a correctly rounded pow: under 32 ulp for x in
+// [1e-5, 1e3].
+//
+// FP32: the hardware log2/exp2 approximations (about 1 ulp each).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sw2d {
+
+constexpr double kLn2Hi = 6.93147180369123816490e-01;
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+constexpr double kInvLn2 = 1.44269504088896338700e+00;
+constexpr double kSqrt2 = 1.41421356237309514547e+00;
+constexpr double kManningExp = 1.3333330;
+
+__device__ __forceinline__ double log_pos(double x)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    if (m > kSqrt2) { m = m * 0.5; e = e + 1; }
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = z * ((2.0 / 3.0) + z * ((2.0 / 5.0) + z * ((2.0 / 7.0) + z * ((2.0 / 9.0) +
+               z * ((2.0 / 11.0) + z * ((2.0 / 13.0) + z * ((2.0 / 15.0) + z * ((2.0 / 17.0) +
+               z * ((2.0 / 19.0) + z * ((2.0 / 21.0) + z * ((2.0 / 23.0) + z * (2.0 / 25.0))))))))))));
+    double hf = 0.5 * f * f;
+    double lm = f - (hf - s * (hf + R));
+    double de = (double)e;
+    return de * kLn2Hi + (de * kLn2Lo + lm);
+}
+
+__device__ __forceinline__ double exp_d(double t)
+{
+    if (t != t) return t;
+    if (t > 709.0) return __longlong_as_double(0x7ff0000000000000LL);
+    if (t < -708.0) return 0.0;
+    double kd = floor(t * kInvLn2 + 0.5);
+    int k = (int)kd;
+    double r = (t - kd * kLn2Hi) - kd * kLn2Lo;
+    double p = 1.0 / 6227020800.0;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    double er = 1.0 + r * (1.0 + r * p);
+    int k1 = k / 2, k2 = k - k1;
+    double s1 = __longlong_as_double((long long)((uint64_t)(k1 + 1023) << 52));
+    double s2 = __longlong_as_double((long long)((uint64_t)(k2 + 1023) << 52));
+    return (er * s1) * s2;
+}
+
+__device__ __forceinline__ double log_d(double x)
+{
+    if (x != x || x < 0.0) return __longlong_as_double(0x7ff8000000000000LL);
+    if (x == 0.0) return -__longlong_as_double(0x7ff0000000000000LL);
+    if (x == __longlong_as_double(0x7ff0000000000000LL)) return x;
+    if (x < 2.2250738585072014e-308)
+        return log_pos(x * 18014398509481984.0) - 54.0 * kLn2Hi - 54.0 * kLn2Lo;
+    return log_pos(x);
+}
+
+// pow(x, 1.3333330) for FP64
+__device__ __forceinline__ double pow1333(double x)
+{
+    if (x != x) return x;
+    if (!(x > 0.0)) return x == 0.0 ? 0.0 : __longlong_as_double(0x7ff8000000000000LL);
+    if (x == __longlong_as_double(0x7ff0000000000000LL)) return x;
+    return exp_d(kManningExp * log_d(x));
+}
+
+// pow(x, 1.3333330) for FP32 (x > 0 on every path that uses the result)
+__device__ __forceinline__ float pow1333(float x)
+{
+    return __builtin_amdgcn_exp2f((float)kManningExp * __builtin_amdgcn_logf(x));
+}
+
+__device__ __forceinline__ float exp_f(float t)
+{
+    return __builtin_amdgcn_exp2f(t * 1.44269504f);
+}
+
+template <typename T> struct num;
+template <> struct num<double> {
+    __device__ static __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
+    __device__ static __forceinline__ double abs_(double x) { return __builtin_fabs(x); }
+    __device__ static __forceinline__ double max_(double a, double b) { return __builtin_fmax(a, b); }
+    __device__ static __forceinline__ double exp_(double x) { return exp_d(x); }
+};
+template <> struct num<float> {
+    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+    __device__ static __forceinline__ float abs_(float x) { return __builtin_fabsf(x); }
+    __device__ static __forceinline__ float max_(float a, float b) { return __builtin_fmaxf(a, b); }
+    __device__ static __forceinline__ float exp_(float x) { return exp_f(x); }
+};
+
+// stream_flow's interpolation (SW2D-EFDC.cu:532-537, 546-551); reads past the
+// end of the series give 0.  Exactly at t = k*dtser it divides 0 by 0, as the
+// reference does.
+__device__ __forceinline__ double series_at(const double *v, int n, int i)
+{
+    return (i >= 0 && i < n) ? v[i] : 0.0;
+}
+__device__ __forceinline__ double series_value(const double *v, int n, double dtser, double t)
+{
+    if (t <= 1.00) return series_at(v, n, 0) * t;
+    int it = (int)(t / dtser);
+    double a = series_at(v, n, it), b = series_at(v, n, it + 1);
+    return a + (b - a) / (dtser * (t - dtser * (it)));
+}
+// point-source table Q[row * nst + i]
+__device__ __forceinline__ double source_value(const double *Q, int nq, int nst, int i, double dtoq,
+                                               double t)
+{
+    if (t <= 1.00) return Q[i] * t;
+    int it = (int)(t / dtoq);
+    double a = (it < nq) ? Q[(long)it * nst + i] : 0.0;
+    double b = (it + 1 < nq) ? Q[(long)(it + 1) * nst + i] : 0.0;
+    return a + (b - a) / (dtoq * (t - dtoq * (it)));
+}
+
+}  // namespace sw2d

@@ -1,0 +1,314 @@
+"""ctypes binding of libsw2d.so (the C ABI of include/sw2d.h).
+
+This is the Python face of the library that the tests and the benchmark use.
+It mirrors the C entry points one to one; there is no computation here and
+no fallback: if libsw2d.so is missing or has no GPU, calls raise.
+"""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsw2d.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "sw2d.h")
+
+FP64, FP32 = 0, 1
+FUSED, PLAIN = 0, 1
+FIELDS = {"h": 0, "um": 1, "vn": 2, "uu": 3, "vv": 4, "uua": 5, "vva": 6, "z": 7, "hm": 8,
+          "hn": 9, "uu1": 10, "vv1": 11}
+
+
+class Sw2dError(RuntimeError):
+    pass
+
+
+class GridDesc(ct.Structure):
+    _fields_ = [("rows", ct.c_int), ("cols", ct.c_int), ("xllcorner", ct.c_double),
+                ("yllcorner", ct.c_double), ("cellsize", ct.c_double), ("nodata", ct.c_double)]
+
+
+class Params(ct.Structure):
+    _fields_ = [("g", ct.c_double), ("manning", ct.c_double), ("th", ct.c_double),
+                ("dt", ct.c_double), ("time0", ct.c_double), ("INT", ct.c_double),
+                ("INF", ct.c_double), ("LWL", ct.c_double), ("EV_WL_min", ct.c_double),
+                ("precision", ct.c_int), ("kernel", ct.c_int), ("adaptive", ct.c_int),
+                ("courant", ct.c_double), ("device", ct.c_int), ("tile_rows", ct.c_int)]
+
+
+class Storm(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in ("peak", "x0", "y0", "vx", "vy", "sigma", "t0", "t1")]
+
+
+class Balance(ct.Structure):
+    _fields_ = [(n, ct.c_double) for n in ("volume", "volume0", "v_source", "v_clip",
+                                           "v_outflow", "q_out", "q_peak", "t_peak", "error")]
+
+
+_D = ct.c_double
+_DP = ct.POINTER(ct.c_double)
+_IP = ct.POINTER(ct.c_int)
+_H = ct.c_void_p
+_lib = None
+
+
+def header_symbols():
+    """Function names declared in include/sw2d.h."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(sw2d_\w+)\s*\(", text, re.M)))
+
+
+def lib():
+    """Load libsw2d.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Sw2dError(f"{LIB_PATH} is missing: build the HIP library first "
+                        "(python sw2d-efdc_amd/build.py)")
+    L = ct.CDLL(LIB_PATH)
+    sig = {
+        "sw2d_default_params": (None, [ct.POINTER(Params)]),
+        "sw2d_version": (ct.c_int, []),
+        "sw2d_last_error": (ct.c_char_p, [_H]),
+        "sw2d_create": (ct.c_int, [ct.POINTER(GridDesc), _DP, _DP, ct.POINTER(Params),
+                                   ct.POINTER(_H)]),
+        "sw2d_destroy": (ct.c_int, [_H]),
+        "sw2d_set_rain_series": (ct.c_int, [_H, _DP, ct.c_int, _D]),
+        "sw2d_set_sources": (ct.c_int, [_H, _DP, _DP, ct.c_int, _DP, ct.c_int, _D]),
+        "sw2d_set_evaporation": (ct.c_int, [_H, _DP, ct.c_int]),
+        "sw2d_evaporation_series": (ct.c_int, [_D, _DP, _DP, ct.c_int, _D, _DP]),
+        "sw2d_set_manning_map": (ct.c_int, [_H, ct.POINTER(ct.c_ubyte), _DP, ct.c_int]),
+        "sw2d_set_storm": (ct.c_int, [_H, ct.POINTER(Storm)]),
+        "sw2d_set_outlet": (ct.c_int, [_H, ct.c_int, ct.c_int]),
+        "sw2d_set_hydrograph": (ct.c_int, [_H, ct.c_long]),
+        "sw2d_set_lake_bc": (ct.c_int, [_H, _IP]),
+        "sw2d_step": (ct.c_int, [_H, ct.c_int]),
+        "sw2d_run_until": (ct.c_int, [_H, _D, ct.c_long, ct.POINTER(ct.c_long)]),
+        "sw2d_sync": (ct.c_int, [_H]),
+        "sw2d_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long), _DP]),
+        "sw2d_get_field": (ct.c_int, [_H, ct.c_int, _DP]),
+        "sw2d_couple_bc": (ct.c_int, [ct.c_int, ct.c_int, _D, _D, _IP, _DP, _DP, _DP, ct.c_int,
+                                      _IP, _IP, _IP, _DP, _IP]),
+        "sw2d_lake_catchment_smooth": (ct.c_int, [ct.c_int, ct.c_int, _IP, _DP]),
+        "sw2d_balance": (ct.c_int, [_H, ct.POINTER(Balance)]),
+        "sw2d_hydrograph": (ct.c_int, [_H, _DP, ct.c_long, ct.POINTER(ct.c_long)]),
+        "sw2d_lake_inflow": (ct.c_int, [_H, _DP, ct.c_int, _IP]),
+        "sw2d_timing": (ct.c_int, [_H, ct.c_int]),
+        "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
+        "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
+        "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_IP)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_math(fn, x, y=None, device=0):
+    """Evaluate the library's device math (0 pow1333, 1 exp, 2 log, 3 sqrt, 4 x/y, 5 fmax)."""
+    x = _f64(np.ravel(x))
+    yy = _f64(np.ravel(y)) if y is not None else None
+    out = np.zeros_like(x)
+    rc = lib().sw2d_device_math(device, fn, _dp(x), _dp(yy) if yy is not None else None,
+                                _dp(out), len(x))
+    if rc:
+        raise Sw2dError(f"sw2d_device_math failed ({rc})")
+    return out
+
+
+def couple_bc(mask, h, vva, uua, dx, dy):
+    """couple_BC_efdc records of a (rows, cols) mask: dict edge -> arrays."""
+    R, C = mask.shape
+    cap = R * C
+    m = np.ascontiguousarray(mask, dtype=np.int32).ravel()
+    cells = np.zeros(4 * cap, np.int32)
+    I = np.zeros(4 * cap, np.int32)
+    J = np.zeros(4 * cap, np.int32)
+    q = np.zeros(4 * cap)
+    counts = np.zeros(4, np.int32)
+    rc = lib().sw2d_couple_bc(R, C, dx, dy, _ip(m), _dp(_f64(h).ravel()), _dp(_f64(vva).ravel()),
+                              _dp(_f64(uua).ravel()), cap, _ip(cells), _ip(I), _ip(J), _dp(q),
+                              _ip(counts))
+    if rc:
+        raise Sw2dError(f"sw2d_couple_bc failed ({rc})")
+    out = {}
+    for k, name in enumerate(("south", "north", "east", "west")):
+        sl = slice(k * cap, k * cap + counts[k])
+        out[name] = dict(cell=cells[sl].copy(), I=I[sl].copy(), J=J[sl].copy(), q=q[sl].copy())
+    return out
+
+
+def lake_catchment_smooth(mask, z):
+    R, C = mask.shape
+    m = np.ascontiguousarray(mask, dtype=np.int32).ravel()
+    zz = _f64(z).ravel().copy()
+    rc = lib().sw2d_lake_catchment_smooth(R, C, _ip(m), _dp(zz))
+    if rc:
+        raise Sw2dError(f"sw2d_lake_catchment_smooth failed ({rc})")
+    return zz.reshape(R, C)
+
+
+def evaporation_series(albedo, T, Rg, dtime):
+    T = _f64(T)
+    Rg = _f64(Rg)
+    out = np.zeros(len(T))
+    rc = lib().sw2d_evaporation_series(albedo, _dp(T), _dp(Rg), len(T), dtime, _dp(out))
+    if rc:
+        raise Sw2dError(f"sw2d_evaporation_series failed ({rc})")
+    return out
+
+
+class Case:
+    """One SW2D case resident on the GPU (a sw2d_handle)."""
+
+    def __init__(self, z, h0, cellsize, *, dt, manning, nodata=-9999.0, xll=0.0, yll=0.0,
+                 g=9.80, th=1.0e-4, INT=0.0, INF=0.0, LWL=0.0, EV_WL_min=0.0, time0=0.0,
+                 precision="fp64", kernel="fused", adaptive=False, courant=0.5, device=0,
+                 tile_rows=0):
+        self._L = lib()
+        z = _f64(z)
+        h0 = _f64(h0)
+        self.R, self.C = z.shape
+        gd = GridDesc(self.R, self.C, xll, yll, cellsize, nodata)
+        p = Params()
+        self._L.sw2d_default_params(ct.byref(p))
+        p.g, p.manning, p.th, p.dt, p.time0 = g, manning, th, dt, time0
+        p.INT, p.INF, p.LWL, p.EV_WL_min = INT, INF, LWL, EV_WL_min
+        p.precision = {"fp64": FP64, "fp32": FP32}[precision]
+        p.kernel = {"fused": FUSED, "plain": PLAIN}[kernel]
+        p.adaptive = 1 if adaptive else 0
+        p.courant = courant
+        p.device = device
+        p.tile_rows = tile_rows
+        self._h = _H()
+        rc = self._L.sw2d_create(ct.byref(gd), _dp(z), _dp(h0), ct.byref(p), ct.byref(self._h))
+        if rc:
+            raise Sw2dError(f"sw2d_create failed ({rc}): "
+                            f"{self._L.sw2d_last_error(None).decode()}")
+        self.nbc = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.sw2d_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise Sw2dError(f"{what} failed ({rc}): {self._L.sw2d_last_error(self._h).decode()}")
+
+    def set_rain(self, rain_mm, dtrain):
+        r = _f64(rain_mm)
+        self._chk(self._L.sw2d_set_rain_series(self._h, _dp(r), len(r), dtrain), "set_rain")
+
+    def set_sources(self, x, y, Q, dtoq):
+        x = _f64(x)
+        y = _f64(y)
+        Q = _f64(Q).reshape(-1, len(x)) if len(x) else np.zeros((0, 0))
+        self._chk(self._L.sw2d_set_sources(self._h, _dp(x), _dp(y), len(x), _dp(Q), Q.shape[0],
+                                           dtoq), "set_sources")
+
+    def set_evaporation(self, evapo_mm):
+        e = _f64(evapo_mm)
+        self._chk(self._L.sw2d_set_evaporation(self._h, _dp(e), len(e)), "set_evaporation")
+
+    def set_manning_map(self, cls, table):
+        c = np.ascontiguousarray(cls, dtype=np.uint8)
+        t = _f64(table)
+        self._chk(self._L.sw2d_set_manning_map(
+            self._h, c.ctypes.data_as(ct.POINTER(ct.c_ubyte)), _dp(t), len(t)), "set_manning_map")
+
+    def set_storm(self, peak, x0, y0, vx, vy, sigma, t0, t1):
+        s = Storm(peak, x0, y0, vx, vy, sigma, t0, t1)
+        self._chk(self._L.sw2d_set_storm(self._h, ct.byref(s)), "set_storm")
+
+    def set_outlet(self, r, c):
+        self._chk(self._L.sw2d_set_outlet(self._h, r, c), "set_outlet")
+
+    def set_hydrograph(self, cap):
+        self._chk(self._L.sw2d_set_hydrograph(self._h, cap), "set_hydrograph")
+
+    def set_lake_bc(self, mask):
+        m = np.ascontiguousarray(mask, dtype=np.int32)
+        self._chk(self._L.sw2d_set_lake_bc(self._h, _ip(m)), "set_lake_bc")
+        n = ct.c_int(0)
+        self._chk(self._L.sw2d_lake_inflow(self._h, None, 0, ct.byref(n)), "lake_inflow")
+        self.nbc = n.value
+
+    def step(self, n=1):
+        self._chk(self._L.sw2d_step(self._h, n), "step")
+
+    def run_until(self, tmax, max_iters=1 << 62):
+        done = ct.c_long(0)
+        self._chk(self._L.sw2d_run_until(self._h, tmax, max_iters, ct.byref(done)), "run_until")
+        return done.value
+
+    def sync(self):
+        self._chk(self._L.sw2d_sync(self._h), "sync")
+
+    def clock(self):
+        t = ct.c_double(0)
+        m = ct.c_long(0)
+        d = ct.c_double(0)
+        self._chk(self._L.sw2d_time(self._h, ct.byref(t), ct.byref(m), ct.byref(d)), "time")
+        return t.value, m.value, d.value
+
+    @property
+    def time(self):
+        return self.clock()[0]
+
+    def field(self, name):
+        out = np.zeros(self.R * self.C)
+        self._chk(self._L.sw2d_get_field(self._h, FIELDS[name], _dp(out)), "get_field")
+        return out.reshape(self.R, self.C)
+
+    def balance(self):
+        b = Balance()
+        self._chk(self._L.sw2d_balance(self._h, ct.byref(b)), "balance")
+        return {n: getattr(b, n) for n, _ in Balance._fields_}
+
+    def hydrograph(self, cap=1 << 20):
+        out = np.zeros(2 * cap)
+        n = ct.c_long(0)
+        self._chk(self._L.sw2d_hydrograph(self._h, _dp(out), cap, ct.byref(n)), "hydrograph")
+        m = min(n.value, cap)
+        return out[: 2 * m].reshape(m, 2)
+
+    def lake_inflow(self):
+        out = np.zeros(max(self.nbc, 1))
+        n = ct.c_int(0)
+        self._chk(self._L.sw2d_lake_inflow(self._h, _dp(out), len(out), ct.byref(n)),
+                  "lake_inflow")
+        return out[: n.value]
+
+    def timing(self, enable=True):
+        self._chk(self._L.sw2d_timing(self._h, 1 if enable else 0), "timing")
+
+    def kernel_time(self):
+        ms = ct.c_double(0)
+        n = ct.c_long(0)
+        self._chk(self._L.sw2d_kernel_time(self._h, ct.byref(ms), ct.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+    def device_bytes(self):
+        b = ct.c_longlong(0)
+        self._chk(self._L.sw2d_device_bytes(self._h, ct.byref(b)), "device_bytes")
+        return b.value
