@@ -224,8 +224,10 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes);
 
 /* Evaluate the library's device math on n inputs (diagnostics and tests):
  * fn 0 pow(x, 1.3333330), 1 exp, 2 log, 3 sqrt, 4 division x/y,
- * 5 fmax(x, y), 6 the rain interpolation of stream_flow with y = time.
- * x, y, out hold n doubles (y may be NULL where unused). */
+ * 5 fmax(x, y), 6 the value of element i-1 within each group of 64
+ * (the kernels' west-neighbour move; 0 for the first), 7 of element i+1
+ * (east-neighbour move; 0 for the last).  x, y, out hold n doubles (y may be
+ * NULL where unused). */
 int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n);
 
 #ifdef __cplusplus

@@ -519,7 +519,7 @@ static void orc_forcing(orc_t *s, double t, orc_forcing_t *fo)
         ql = ql / (s->dx * s->dy);
         s->ql[s->src[i]] = ql;
     }
-    fo->rr = orc_rain_rate(s->rain, s->nrain, s->dtrain, t);
+    fo->rr = (s->nrain > 0) ? orc_rain_rate(s->rain, s->nrain, s->dtrain, t) : 0.0;  /* no rain.dat series: no rain */
     fo->Ev = 0.0;
     if (s->evap_on) fo->Ev = orc_series(s->evapo, s->nevapo, s->dtrain, t) / (s->dtrain * 1000.0);
     fo->storm = s->storm_on && (t >= s->st_t0) && (t < s->st_t1);
@@ -645,6 +645,8 @@ static double orc_cfl_min(const orc_t *s)
 {
     const int C = s->C, R = s->R;
     double m = INFINITY;
+    /* a minimum does not depend on the order it is taken in */
+#pragma omp parallel for schedule(static) num_threads(s->nthreads) reduction(min : m)
     for (int r = 0; r < R; r++)
         for (int j = 0; j < C; j++) {
             int c = r * C + j;

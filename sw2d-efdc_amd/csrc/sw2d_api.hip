@@ -704,9 +704,9 @@ namespace {
 __global__ void math_probe(int fn, const double *x, const double *y, double *out, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double a = x[i];
-    const double b = y ? y[i] : 0.0;
+    // every lane stays active (the cross-lane moves need the whole wave)
+    const double a = (i < n) ? x[i] : 0.0;
+    const double b = (y && i < n) ? y[i] : 0.0;
     double r = 0;
     switch (fn) {
     case 0: r = pow1333(a); break;
@@ -715,15 +715,17 @@ __global__ void math_probe(int fn, const double *x, const double *y, double *out
     case 3: r = __builtin_sqrt(a); break;
     case 4: r = a / b; break;
     case 5: r = __builtin_fmax(a, b); break;
+    case 6: r = from_west(a); break;  // lane layout of the cross-lane moves
+    case 7: r = from_east(a); break;
     default: r = 0; break;
     }
-    out[i] = r;
+    if (i < n) out[i] = r;
 }
 }  // namespace
 
 extern "C" int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n)
 {
-    if (!x || !out || n < 0 || fn < 0 || fn > 5) return SW2D_E_ARG;
+    if (!x || !out || n < 0 || fn < 0 || fn > 7) return SW2D_E_ARG;
     if ((fn == 4 || fn == 5) && !y) return SW2D_E_ARG;
     if (n == 0) return SW2D_OK;
     int ndev = 0;

@@ -125,13 +125,45 @@ struct PitchedState {
     }
 };
 
-// Bookkeeping of iteration k, run by one wave: advance the clock ring, reduce
-// the partial sums of iteration k-1 into the ledger (fixed order), record the
-// hydrograph sample of iteration k-1, integrate the lake inflow of the state
-// at the start of iteration k.  uua/vva are evaluated by the functor F.
-template <typename F>
-__device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &vel, const int lane)
+// Bookkeeping of iteration k.  Part 0 (one wave): advance the clock ring,
+// reduce the partial sums of iteration k-1 into the ledger (fixed order) and
+// record its hydrograph sample.  Part b >= 1 (one wave each): integrate the
+// lake inflow of records 64(b-1) .. 64b-1 over the state at the start of
+// iteration k.  uua/vva are evaluated by the functor F.
+__device__ __forceinline__ void reduce_partials(const Partial *p, int n, int lane, double &vol,
+                                                double &src, double &clip, double &qo)
 {
+    vol = 0; src = 0; clip = 0; qo = 0;
+    // four independent loads in flight per lane
+    int i = lane;
+    for (; i + 192 < n; i += 256) {
+        const Partial a = p[i], b = p[i + 64], c = p[i + 128], d = p[i + 192];
+        vol += a.vol; src += a.src; clip += a.clip; qo += a.qout;
+        vol += b.vol; src += b.src; clip += b.clip; qo += b.qout;
+        vol += c.vol; src += c.src; clip += c.clip; qo += c.qout;
+        vol += d.vol; src += d.src; clip += d.clip; qo += d.qout;
+    }
+    for (; i < n; i += 64) {
+        const Partial a = p[i];
+        vol += a.vol; src += a.src; clip += a.clip; qo += a.qout;
+    }
+    vol = wave_sum(vol); src = wave_sum(src); clip = wave_sum(clip); qo = wave_sum(qo);
+}
+
+template <typename F>
+__device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &vel, const int part,
+                            const int lane)
+{
+    if (part > 0) {
+        const int i = (part - 1) * 64 + lane;
+        if (i < s.nbc) {
+            const int r = s.bc_r[i], j = s.bc_c[i];
+            double hh, ua, va;
+            vel(r, j, hh, ua, va);
+            s.bc_acc[i] += bc_rate(s.bc_kind[i], hh, ua, va, s.dx, s.dy) * st.dt2;
+        }
+        return;
+    }
     if (lane == 0) {
         Clock &nb = s.clk[s.slot_b];
         nb.t = st.tm + st.dt;
@@ -140,12 +172,8 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
         s.clk[s.slot_c].cfl_bits = kInfBits;
     }
     if (s.step_index > 0) {
-        double vol = 0, src = 0, clip = 0, qo = 0;
-        for (int i = lane; i < s.nparts; i += 64) {
-            Partial p = s.part_in[i];
-            vol += p.vol; src += p.src; clip += p.clip; qo += p.qout;
-        }
-        vol = wave_sum(vol); src = wave_sum(src); clip = wave_sum(clip); qo = wave_sum(qo);
+        double vol, src, clip, qo;
+        reduce_partials(s.part_in, s.nparts, lane, vol, src, clip, qo);
         if (lane == 0) {
             Ledger &L = *s.ledger;
             double dt2p = s.clk[s.slot_a].dt2_prev;
@@ -163,12 +191,6 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
             }
         }
     }
-    for (int i = lane; i < s.nbc; i += 64) {
-        int r = s.bc_r[i], j = s.bc_c[i];
-        double hh, ua, va;
-        vel(r, j, hh, ua, va);
-        s.bc_acc[i] += bc_rate(s.bc_kind[i], hh, ua, va, s.dx, s.dy) * st.dt2;
-    }
 }
 
 // Ledger view for the host: the committed ledger plus the partials of the
@@ -176,12 +198,7 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
 __device__ __forceinline__ void ledger_peek(const StepCommon &s, Ledger *out, const int lane)
 {
     double vol = 0, src = 0, clip = 0, qo = 0;
-    if (s.step_index > 0)
-        for (int i = lane; i < s.nparts; i += 64) {
-            Partial p = s.part_in[i];
-            vol += p.vol; src += p.src; clip += p.clip; qo += p.qout;
-        }
-    vol = wave_sum(vol); src = wave_sum(src); clip = wave_sum(clip); qo = wave_sum(qo);
+    if (s.step_index > 0) reduce_partials(s.part_in, s.nparts, lane, vol, src, clip, qo);
     if (lane == 0) {
         Ledger L = *s.ledger;
         if (s.step_index > 0) {

@@ -30,9 +30,6 @@
 
 namespace sw2d {
 
-template <typename T> __device__ __forceinline__ T from_east(T v) { return __shfl_down(v, 1, 64); }
-template <typename T> __device__ __forceinline__ T from_west(T v) { return __shfl_up(v, 1, 64); }
-
 // bit of lane-1 of a wave-wide predicate
 __device__ __forceinline__ bool west_bit(bool p, int lane)
 {
@@ -50,14 +47,14 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     const int lane = threadIdx.x;
     const StepTime st = step_time(s);
 
-    if ((int)blockIdx.x == a.ntiles) {
+    if ((int)blockIdx.x >= a.ntiles) {
         PitchedState<T> ps{a.h_in, a.um_in, a.vn_in, s.R, s.C, a.P, (T)s.th};
         auto vel = [&](int r, int j, double &hh, double &ua, double &va) {
             hh = (double)ps.hv(r, j);
             ua = (double)ps.uua(r, j);
             va = (double)ps.vva(r, j);
         };
-        bookkeeping(s, st, vel, lane);
+        bookkeeping(s, st, vel, (int)blockIdx.x - a.ntiles, lane);
         return;
     }
 
@@ -185,8 +182,10 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         // ---- fluxes of row f = k-1 (flux, SW2D-EFDC.cu:97-353) ----
         const int f = k - 1;
         T umn = 0, vnn = 0;
+        // rows before r0 - 1 feed no output row: the warm-up only builds the window
+        const bool flux_um = k >= r0, flux_vn = k >= r0 && k <= r1 + 1;
         // x direction: um at the north face of (f, j); w = row f-1, e = row f
-        if (f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th))) {
+        if (flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th))) {
             const T hhe = h1 + z1, hhw = h2 + z2, hhep = h1 - th, hhwp = h2 - th;
             if (hhe < z2) {
                 umn = (h2 > th) ? T(0.5440) * h2 * M::sqrt_(gg * h2) : T(0);
@@ -227,7 +226,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
             }
         }
         // y direction: vn at the west face of (f, j); s = column j-1, n = column j
-        if (j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th))) {
+        if (flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th))) {
             const T hhn = h1 + z1, hhs = h1w + z1w, hhnp = h1 - th, hhsp = h1w - th;
             if (hhn < z1w) {
                 vnn = (h1w > th) ? T(0.5440) * h1w * M::sqrt_(gg * h1w) : T(0);
@@ -270,9 +269,11 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
 
         // ---- first continuity of row g = k-2 at time t (602-664) ----
         const int g = k - 2;
+        bool dryg = false;
+        T umpg = 0, vnpg = 0;
+        if (k >= r0 + 1) {
         const T vnnPe = from_east(vnnP);
         T h1v;
-        bool dryg;
         {
             const bool wet = h2 > ewl;
             const T Pc = wet ? P_lake : P_land;
@@ -300,19 +301,20 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
             dryg = a2 && (h1v < th);
         }
         // ---- treat_error on the faces of row g (695-725) ----
-        T umpg = umnP;
+        umpg = umnP;
         if (dryP && umpg > 0) umpg = 0;
         if (dryg && umpg < 0) umpg = 0;
         const bool drygw = west_bit(dryg, lane);
-        T vnpg = vnnP;
+        vnpg = vnnP;
         if (drygw && vnpg > 0) vnpg = 0;
         if (dryg && vnpg < 0) vnpg = 0;
         if (is_out_col && g == s.out_r && dryg) qo_eff = 0;
+        }
 
         // ---- second continuity of row q = k-3 at time t + dt; store ----
         const int q = k - 3;
+        if (k >= r0 + 3) {
         const T vnpPe = from_east(vnpP);
-        {
             const bool wet = h3 > ewl;
             const T Pc = wet ? P_lake : P_land;
             const T E = wet ? (T)ev2 : T(0.000);
@@ -466,7 +468,8 @@ static hipError_t launch_fused_t(sw2d_handle *h, const StepCommon &sc)
     a.nstrips = h->nstrips; a.ntiles = h->ntiles;
     a.xcd_remap = 1;
     a.s = sc;
-    dim3 grid(h->ntiles + 1), block(64);
+    // one wave per tile, one for clock + ledger, one per 64 lake-inflow records
+    dim3 grid(h->ntiles + 1 + (h->nbc + 63) / 64), block(64);
     if (h->prm.adaptive)
         hipLaunchKernelGGL((fused_step<T, true>), grid, block, 0, h->stream, a);
     else

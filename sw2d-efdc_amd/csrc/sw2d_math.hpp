@@ -142,4 +142,36 @@ __device__ __forceinline__ double source_value(const double *Q, int nq, int nst,
     return a + (b - a) / (dtoq * (t - dtoq * (it)));
 }
 
+// Cross-lane moves along the wavefront with DPP (GFX9 wave_shr:1 / wave_shl:1).
+// from_west: lane i receives lane i-1 (lane 0 receives 0); from_east: lane i
+// receives lane i+1 (lane 63 receives 0).  All lanes must be active.
+__device__ __forceinline__ int dpp_from_west(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int dpp_from_east(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ double from_west(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_west((int)(b & 0xffffffffLL)), hi = dpp_from_west((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ double from_east(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_from_east((int)(b & 0xffffffffLL)), hi = dpp_from_east((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ __forceinline__ float from_west(float v)
+{
+    return __int_as_float(dpp_from_west(__float_as_int(v)));
+}
+__device__ __forceinline__ float from_east(float v)
+{
+    return __int_as_float(dpp_from_east(__float_as_int(v)));
+}
+
 }  // namespace sw2d

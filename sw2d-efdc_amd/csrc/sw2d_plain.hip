@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(64) plain_bookkeeping(const PlainArgs a)
         const int p = r * C + j;
         hh = a.h[p]; ua = a.uua[p]; va = a.vva[p];
     };
-    bookkeeping(s, st, vel, threadIdx.x);
+    bookkeeping(s, st, vel, (int)blockIdx.x, threadIdx.x);
 }
 
 // init_inf + initiald (SW2D-EFDC.cu:29-91) and the initial CFL minimum
@@ -365,7 +365,7 @@ hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc)
     const PlainArgs a = plain_args(h, sc);
     const dim3 grid(h->plain_blocks), block(256);
     hipStream_t st = h->stream;
-    hipLaunchKernelGGL(plain_bookkeeping, dim3(1), dim3(64), 0, st, a);
+    hipLaunchKernelGGL(plain_bookkeeping, dim3(1 + (h->nbc + 63) / 64), dim3(64), 0, st, a);
     if (h->prm.adaptive) hipLaunchKernelGGL(plain_flux<true>, grid, block, 0, st, a);
     else hipLaunchKernelGGL(plain_flux<false>, grid, block, 0, st, a);
     hipLaunchKernelGGL(plain_continuity<0>, grid, block, 0, st, a);

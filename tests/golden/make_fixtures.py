@@ -1,0 +1,41 @@
+"""Write the test fixtures derived from the reference's example case data.
+
+Reads db/COUPLED_peri_lake/SW2D/input/{mask.dat, initial_condition.dat} of
+LabHidro/SW2D-EFDC (a checkout at /root/reference by default) and stores them
+as compressed arrays in tests/golden/peri_lake.npz.  These are input data
+files of the reference's only example case (556 x 701, dx = 10 m); its DEM is
+not part of the repository, so no flow results exist to compare against.
+"""
+import pathlib
+import sys
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+
+
+def read_esri(path, dtype):
+    with open(path) as f:
+        hdr = {}
+        for _ in range(6):
+            k, v = f.readline().split()
+            hdr[k.lower()] = float(v)
+        vals = np.array(f.read().split(), dtype=np.float64)
+    return hdr, vals.astype(dtype)
+
+
+def main(ref="/root/reference"):
+    case = pathlib.Path(ref) / "db" / "COUPLED_peri_lake" / "SW2D" / "input"
+    hdr, mask = read_esri(case / "mask.dat", np.int32)
+    R, C = int(hdr["nrows"]), int(hdr["ncols"])
+    mask = mask[: R * C].reshape(R, C)
+    h0 = np.array(open(case / "initial_condition.dat").read().split(), dtype=np.float64)
+    h0 = h0[: R * C].reshape(R, C)
+    np.savez_compressed(HERE / "peri_lake.npz", mask=mask, h0=h0,
+                        header=np.array([hdr["ncols"], hdr["nrows"], hdr["xllcorner"],
+                                         hdr["yllcorner"], hdr["cellsize"], hdr["nodata_value"]]))
+    print(R, C, int((mask == 1).sum()), h0.max())
+
+
+if __name__ == "__main__":
+    main(*sys.argv[1:])

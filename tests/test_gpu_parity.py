@@ -56,6 +56,18 @@ def test_device_math_bitwise(pkg, po):
     assert np.array_equal(sw.device_math(5, nanx, np.zeros(4)), np.array([0.0, 1.0, 0.0, 0.0]))
 
 
+def test_cross_lane_moves(pkg):
+    """The west/east neighbour moves of the fused kernel: element i receives
+    element i-1 (resp. i+1) of its group of 64; the group's first (last) gets 0."""
+    sw = pkg.sw2d
+    x = np.arange(1, 257, dtype=np.float64) * 1.5 + 1e-300
+    lane = np.arange(256) % 64
+    west = np.where(lane == 0, 0.0, np.roll(x, 1))
+    east = np.where(lane == 63, 0.0, np.roll(x, -1))
+    assert np.array_equal(sw.device_math(6, x), west)
+    assert np.array_equal(sw.device_math(7, x), east)
+
+
 def pair(pkg, po, z, h0, dx, *, kernel="fused", tile_rows=0, dt=0.5, manning=0.03, **kw):
     g = pkg.Case(z, h0, dx, dt=dt, manning=manning, kernel=kernel, tile_rows=tile_rows, **kw)
     o = po.Oracle(z, h0, dx, dt=dt, manning=manning,

@@ -1,0 +1,296 @@
+"""Pinning of the CPU FP64 oracle (oracle/sw2d_oracle.c) -- CPU only.
+
+The reference holds no tests, fixtures or results for the SW2D path, so each
+oracle function is pinned by what the mathematics fixes: library routines
+(libm), closed forms, hand-computed branch values, invariants, and the
+reference's own example input data (tests/golden/peri_lake.npz).
+"""
+import math
+import pathlib
+
+import numpy as np
+import pytest
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+G = 9.80
+
+
+def ulps(a, b):
+    return np.abs(a - b) / np.spacing(np.abs(b))
+
+
+def test_pow1333_against_libm(po):
+    rng = np.random.default_rng(1)
+    x = np.concatenate([10 ** rng.uniform(-5, 3, 100000), [1e-4, 0.5, 1.0, 2.0, 1000.0]])
+    got = po.pow1333(x)
+    ref = np.power(x, 1.3333330)
+    assert ulps(got, ref).max() <= 32
+    assert po.pow1333([0.0])[0] == 0.0
+    assert math.isnan(po.pow1333([-1.0])[0])
+
+
+def test_exp_log_against_libm(po):
+    L = po.lib()
+    rng = np.random.default_rng(2)
+    t = rng.uniform(-700, 700, 50000)
+    e = np.array([L.orc_exp(v) for v in t])
+    assert ulps(e, np.exp(t)).max() <= 2
+    x = 10 ** rng.uniform(-300, 300, 50000)
+    lg = np.array([L.orc_log(v) for v in x])
+    assert ulps(lg, np.log(x)).max() <= 2
+    assert L.orc_exp(0.0) == 1.0 and L.orc_log(1.0) == 0.0
+
+
+def test_forcing_interpolation_known_values(po):
+    """stream_flow's rain term (SW2D-EFDC.cu:546-553), hand-evaluated."""
+    rain, dtr = np.array([25.0, 0.0, 0.0, 0.0]), 1800.0
+    # ramp for t <= 1: rain[0] * t
+    assert po.rain_rate(rain, dtr, 0.0) == 0.0
+    assert po.rain_rate(rain, dtr, 0.5) == pytest.approx(12.5 / 1.8e6, rel=1e-15)
+    assert po.rain_rate(rain, dtr, 1.0) == pytest.approx(25.0 / 1.8e6, rel=1e-15)
+    # t = 2: 25 + (0 - 25) / (1800 * 2) = 24.99305555...
+    assert po.series(rain, dtr, 2.0) == pytest.approx(24.993055555555557, rel=1e-15)
+    # exactly at t = 1800 the reference divides 0 by 0
+    assert math.isnan(po.series(rain, dtr, 1800.0))
+    # with unequal neighbours it divides a nonzero by 0: 10 -> 0 at t = 1800,
+    # 0 -> 5 at t = 3600
+    r2 = np.array([25.0, 10.0, 0.0, 5.0])
+    assert po.series(r2, dtr, 1800.0) == -math.inf
+    assert po.series(r2, dtr, 3600.0) == math.inf
+    # between boundaries with equal neighbours the value is exact
+    assert po.series(rain, dtr, 2000.0) == 0.0
+    # past the end of the series the missing values read as 0
+    assert po.series(np.array([3.0]), 10.0, 25.0) == 0.0
+
+
+def test_source_index_example_case(po):
+    """coord_source_sink.dat of the example case: (4921.04, 5215.8) m on the
+    556 x 701, 10 m grid with origin (0, 0) -> qinx 492, qiny 701 - 522 = 179,
+    flat index qiny * cols + qinx + 1."""
+    idx = po.source_index(701, 556, 0.0, 0.0, 10.0, 4921.04, 5215.8)
+    assert idx == 179 * 556 + 492 + 1
+
+
+def test_evaporation_physical_values(po):
+    """Energy-balance evaporation at 20 C: water density 998.2 kg/m3 (ITS-90)
+    and latent heat 2.4535 MJ/kg (Henderson-Sellers 1984)."""
+    E = po.evaporation(0.2, [20.0], [500.0], 3600.0)[0]
+    expect = 400.0 / (2.4535e6 * 998.207) * 3600.0 * 1000.0
+    assert E == pytest.approx(expect, rel=2e-4)
+    assert po.evaporation(0.2, [20.0], [-50.0], 3600.0)[0] == 0.0
+
+
+def two_cell(po, z, h, dt=0.5, n=0.03, **kw):
+    """A 2 x 2 grid whose second column is NODATA: one row face between
+    (0, 0) (north) and (1, 0) (south)."""
+    zz = np.array([[z[0], -9999.0], [z[1], -9999.0]])
+    hh = np.array([[h[0], 0.0], [h[1], 0.0]])
+    return po.Oracle(zz, hh, 10.0, dt=dt, manning=n, **kw)
+
+
+def test_dry_bed_weir_branch_1(po):
+    """Surface of the south cell below the bed of the north cell: broad-crested
+    weir 0.544 h sqrt(g h) from the north depth (SW2D-EFDC.cu:129-136)."""
+    o = two_cell(po, (1.0, 0.0), (0.25, 0.5))
+    o.step(1)
+    assert o.field("um")[1, 0] == 0.544 * 0.25 * math.sqrt(G * 0.25)
+
+
+def test_dry_bed_weir_branch_2(po):
+    """Mirror case: north surface below the south bed, flux northwards."""
+    o = two_cell(po, (0.0, 1.0), (0.5, 0.25))
+    o.step(1)
+    assert o.field("um")[1, 0] == -0.544 * 0.25 * math.sqrt(G * 0.25)
+
+
+def test_dry_bed_weir_branch_3(po):
+    """Depths straddling th: -sgn * 0.35 * hh3 * sqrt(2 g hh3) with
+    hh3 = max surface - max bed = 0.2 - 0.1 (SW2D-EFDC.cu:148-158)."""
+    o = two_cell(po, (0.0, 0.1), (0.2, 0.00005))
+    o.step(1)
+    assert o.field("um")[1, 0] == pytest.approx(0.35 * 0.1 * math.sqrt(2 * G * 0.1), rel=1e-15)
+
+
+def test_treat_error_zeroes_outflow_of_drying_cell(po):
+    """A shallow south cell (1.2e-4 m) drains through a weir face; with a long
+    step its first-continuity depth falls below th, so treat_error zeroes the
+    face and the second continuity leaves both depths unchanged."""
+    o = two_cell(po, (0.0, 1.0), (0.0, 1.2e-4), dt=100.0)
+    o.step(1)
+    assert o.field("um")[1, 0] == 0.0
+    h = o.field("h")
+    assert h[1, 0] == 1.2e-4 and h[0, 0] == 0.0
+    # deeper and shorter (dt2 = 20 s): no drying, the weir flux stands and
+    # moves water north
+    o = two_cell(po, (0.0, 1.0), (0.0, 0.01), dt=10.0)
+    o.step(1)
+    assert o.field("um")[1, 0] == -0.544 * 0.01 * math.sqrt(G * 0.01)
+    assert o.field("h")[0, 0] > 0.0
+
+
+def dyadic_lake(R=12, C=9, seed=3):
+    rng = np.random.default_rng(seed)
+    z = rng.integers(0, 40, size=(R, C)) / 8.0     # bed on a 1/8 m grid
+    return z, 10.0 - z                             # flat surface at 10 m
+
+
+def test_lake_at_rest(po):
+    z, h = dyadic_lake()
+    o = po.Oracle(z, h, 10.0, dt=0.5, manning=0.03)
+    o.step(50)
+    assert np.all(o.field("um") == 0) and np.all(o.field("vn") == 0)
+    assert np.array_equal(o.field("h"), h)
+
+
+def test_uniform_rain_on_flat_closed_box(po):
+    """Flat closed box, constant rain series r mm per dtrain: every cell holds
+    h_K = dt2 * rr * (K - 1/2) after K iterations (the second continuity runs
+    at t + dt; t + dt = 0.5 is inside the start-up ramp rain[0] * t)."""
+    R, C, r, dtr = 7, 11, 12.0, 600.0
+    o = po.Oracle(np.zeros((R, C)), np.zeros((R, C)), 10.0, dt=0.5, manning=0.03)
+    o.set_rain(np.full(10, r), dtr)
+    K = 400
+    o.step(K)
+    rr = r / (dtr * 1000.0)
+    h = o.field("h")
+    assert np.all(h == h[0, 0])
+    assert h[0, 0] == pytest.approx(1.0 * rr * (K - 0.5), rel=1e-12)
+    led = o.ledger()
+    assert led["volume"] == pytest.approx(led["v_src"], rel=1e-12)
+
+
+def test_mass_conservation_closed_walls(po, cases):
+    R, C = 40, 33
+    z = cases.fbm(R, C, 0.7, 1.0, 4)
+    h = np.zeros((R, C))
+    h[10:20, 8:18] = 0.8
+    o = po.Oracle(z, h, 5.0, dt=0.25, manning=0.03)
+    v0 = h.sum()
+    o.step(600)
+    led = o.ledger()
+    assert led["v_clip"] == 0.0
+    assert o.field("h").sum() == pytest.approx(v0, rel=1e-13)
+    assert np.abs(o.field("um")).max() > 0      # water did move
+
+
+def test_kinematic_plane_steady_outflow(po):
+    """Steady rain on a plane draining to a free-outflow outlet: the outlet
+    discharge approaches rain * area (the ledger closes on the way)."""
+    R, C, dx = 20, 3, 10.0
+    r = np.arange(R, dtype=np.float64)[:, None]
+    z = np.broadcast_to(0.05 * dx * (R - 1 - r), (R, C)).copy()
+    z[:, 0] = 50.0
+    z[:, 2] = 50.0                        # walls: a one-cell-wide channel
+    o = po.Oracle(z, np.zeros((R, C)), dx, dt=0.25, manning=0.03)
+    rain = np.full(40, 100.0)             # mm per 600 s
+    o.set_rain(rain, 600.0)
+    o.set_outlet(R - 1, 1)
+    o.step(40000)
+    led = o.ledger()
+    area_rate = 100.0 / 600.0 / 1000.0 * dx * dx * R * C
+    assert led["q_out"] == pytest.approx(area_rate, rel=0.02)
+    err = led["volume"] - (led["v0"] + led["v_src"] + led["v_clip"] - led["v_out"])
+    assert abs(err) <= 1e-9 * led["v_src"]
+
+
+def test_adaptive_step_lake_at_rest(po):
+    """At rest, dt2 = courant * dx / sqrt(g * hmax) on the deepest face, capped
+    at dt2_max."""
+    z, h = dyadic_lake()
+    o = po.Oracle(z, h, 10.0, dt=5.0, manning=0.03)
+    o.set_adaptive(0.5, 10.0)
+    hm_max = max((0.5 * (h[1:, :] + h[:-1, :])).max(), (0.5 * (h[:, 1:] + h[:, :-1])).max())
+    assert o.dt2_next == 0.5 * 10.0 / math.sqrt(G * hm_max)
+    o.step(3)
+    assert o.time == pytest.approx(3 * 0.5 * 10.0 / math.sqrt(G * hm_max), rel=1e-15)
+    o2 = po.Oracle(z, h, 10.0, dt=0.1, manning=0.03)
+    o2.set_adaptive(0.5, 0.2)
+    assert o2.dt2_next == 0.2
+
+
+def test_outlet_weir_discharge(po):
+    """One wet cell on the south edge with the outlet: the first iteration's
+    discharge is 0.5440 h sqrt(g h) * dy from the starting depth."""
+    z = np.zeros((3, 3))
+    h = np.zeros((3, 3))
+    h[2, 1] = 0.3
+    o = po.Oracle(z, h, 10.0, dt=0.5, manning=0.03)
+    o.set_outlet(2, 1)
+    o.set_hydrograph(10)
+    o.step(1)
+    q = 0.5440 * 0.3 * math.sqrt(G * 0.3) * 10.0
+    hg = o.hydrograph()
+    assert hg.shape == (1, 2) and hg[0, 0] == 1.0 and hg[0, 1] == q
+    led = o.ledger()
+    assert led["v_out"] == q * 1.0
+    assert led["volume"] == pytest.approx(0.3 * 100.0 - q, rel=1e-12)
+
+
+def numpy_couple_records(mask):
+    """couple_BC_efdc's record sets written independently with numpy: for
+    every catchment cell (mask != 1) the differing 4-neighbours it tests."""
+    R, C = mask.shape
+    recs = {"south": set(), "north": set(), "east": set(), "west": set()}
+    cat = mask != 1
+    ini = (R - 1) - np.arange(R)[:, None] * np.ones((1, C), int)   # scan row index per cell
+    inj = np.ones((R, 1), int) * np.arange(C)[None, :]
+    mid_r = (ini > 0) & (ini < R - 1)
+    mid_c = (inj > 0) & (inj < C - 1)
+    up = np.zeros_like(cat); up[1:, :] = mask[1:, :] != mask[:-1, :]        # p - C
+    dn = np.zeros_like(cat); dn[:-1, :] = mask[:-1, :] != mask[1:, :]       # p + C
+    lf = np.zeros_like(cat); lf[:, 1:] = mask[:, 1:] != mask[:, :-1]        # p - 1
+    rt = np.zeros_like(cat); rt[:, :-1] = mask[:, :-1] != mask[:, 1:]       # p + 1
+    tS = cat & up & ((mid_r) | (mid_c & (ini == 0)))
+    tN = cat & dn & ((mid_r) | (mid_c & (ini == R - 1)))
+    tE = cat & lf & ((mid_r & (mid_c | (inj == C - 1))) | (mid_c & ((ini == 0) | (ini == R - 1))))
+    tW = cat & rt & ((mid_r & (mid_c | (inj == 0))) | (mid_c & ((ini == 0) | (ini == R - 1))))
+    flat = np.arange(R * C).reshape(R, C)
+    for name, t in (("south", tS), ("north", tN), ("east", tE), ("west", tW)):
+        recs[name] = set(flat[t].tolist())
+    return recs
+
+
+def test_couple_bc_example_mask(po, pkg):
+    """Records of couple_BC_efdc on the example case's lake mask (556 x 701,
+    50638 lake cells): oracle, library and an independent numpy formulation
+    agree; EFDC (I, J) follow the reference's per-edge offsets."""
+    d = np.load(GOLDEN / "peri_lake.npz")
+    mask = d["mask"]
+    R, C = mask.shape
+    assert (mask == 1).sum() == 50638
+    rng = np.random.default_rng(0)
+    h = rng.random((R, C))
+    uua = rng.standard_normal((R, C))
+    vva = rng.standard_normal((R, C))
+    ro = po.couple_bc(mask, h, vva, uua, 10.0, 10.0)
+    rl = pkg.couple_bc(mask, h, vva, uua, 10.0, 10.0)
+    ind = numpy_couple_records(mask)
+    for k in ("south", "north", "east", "west"):
+        assert len(ro[k]["cell"]) > 0
+        for f in ("cell", "I", "J", "q"):
+            assert np.array_equal(ro[k][f], rl[k][f]), (k, f)
+        assert set(ro[k]["cell"].tolist()) == ind[k], k
+        cells = ro[k]["cell"]
+        ini = (R - 1) - cells // C
+        inj = cells % C
+        off = {"south": (1, 2), "north": (1, 0), "east": (0, 1), "west": (2, 1)}[k]
+        assert np.array_equal(ro[k]["I"], inj + off[0])
+        assert np.array_equal(ro[k]["J"], ini + off[1])
+        # scan order: south edge first, west to east
+        assert np.all(np.diff(ini * C + inj) > 0)
+    q = ro["north"]["q"]
+    c = ro["north"]["cell"]
+    u = uua.ravel()[c]
+    assert np.array_equal(q, np.where(u > 0, u * h.ravel()[c] * 10.0, 0.0))
+
+
+def test_lake_catchment_smooth_example_mask(po, pkg, cases):
+    d = np.load(GOLDEN / "peri_lake.npz")
+    mask = d["mask"]
+    z = 20.0 + cases.fbm(*mask.shape, 0.7, 5.0, 9)
+    z[0, :] = -9999.0
+    a = po.lake_catchment_smooth(mask, z)
+    b = pkg.lake_catchment_smooth(mask, z)
+    assert np.array_equal(a, b)
+    changed = a != z
+    assert changed.sum() > 0 and np.all(mask[changed] != 1)
