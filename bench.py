@@ -45,9 +45,10 @@ def load_package():
 
 def bytes_per_cell(precision, has_class_map):
     """Compulsory HBM bytes of one cell-update of the fused kernel: read h, um,
-    vn, z, write h, um, vn (+1 byte Manning class when a class map is used)."""
+    vn and the FP64 bed z, write h, um, vn (+1 byte Manning class when a class
+    map is used): 57 B in FP64, 33 B in the FP32 form."""
     e = 8 if precision == "fp64" else 4
-    return 7 * e + (1 if has_class_map else 0)
+    return 6 * e + 8 + (1 if has_class_map else 0)
 
 
 def make_case(pkg, spec, precision, kernel, tile_rows):
@@ -115,7 +116,11 @@ def main():
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--kernel", default="fused", choices=["fused", "plain"])
-    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--tile-rows", type=int, default=0,
+                    help="rows per tile of the fused kernel (0 = measured by sw2d_autotune)")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="after the run, copy this many MB device-to-device with torch (a plain "
+                         "copy kernel of known size, to scale rocprofv3 FETCH/WRITE_SIZE)")
     ap.add_argument("--plain-steps", type=int, default=200,
                     help="iterations of the plain FP64 kernel sequence for the comparison "
                          "line (0 = skip)")
@@ -136,6 +141,9 @@ def main():
     cells = R * C
 
     case = make_case(pkg, spec, args.precision, args.kernel, args.tile_rows)
+    tile_rows = args.tile_rows
+    if args.kernel == "fused" and tile_rows == 0:
+        tile_rows = case.autotune(10)
     case.step(args.warmup)
     case.sync()
 
@@ -180,7 +188,7 @@ def main():
             "grid": [R, C],
             "kernel": args.kernel,
             "precision": args.precision,
-            "tile_rows": args.tile_rows,
+            "tile_rows": tile_rows,
             "simulated_s_after_run": t_sim,
             "dt2_last_s": dt2,
             "device_bytes": case.device_bytes(),
@@ -219,6 +227,12 @@ def main():
         out["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, threads)
     else:
         out["cpu_baseline"] = None
+    if args.calibrate_copy_mb > 0:
+        # the library's lane-move probe reads n doubles and writes n doubles
+        # (sw2d_device_math fn 6): a streaming kernel of known size
+        n = args.calibrate_copy_mb * (1 << 20) // 8
+        pkg.sw2d.device_math(6, np.ones(n))
+        out["copy_calibration_bytes"] = {"kernel": "math_probe", "read": 8 * n, "write": 8 * n}
     print(json.dumps(out))
 
 

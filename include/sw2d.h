@@ -168,6 +168,13 @@ int sw2d_set_hydrograph(sw2d_handle *h, long cap);
  * integrated over every iteration on the device. */
 int sw2d_set_lake_bc(sw2d_handle *h, const int *mask);
 
+/* Extension (the project's autotuner): time the fused kernel for each
+ * candidate tile height (rows per one-wave tile) over iters launches on the
+ * current state and keep the fastest; *tile_rows receives it (0 for the
+ * plain kernels).  Results do not depend on the tile height.  Only valid
+ * before the first iteration.  Synchronises. */
+int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows);
+
 /* Advance n loop iterations of the reference's call sequence (4798-4877):
  * flux, continuity(t), treat_error, continuity(t + dt), derived fields.
  * Asynchronous: returns once the work is queued. */

@@ -55,6 +55,7 @@ def lib():
         L.orc_set_adaptive.argtypes = [ct.c_void_p, _D, _D]
         L.orc_set_bc.argtypes = [ct.c_void_p, _IP, _IP, ct.c_int]
         L.orc_step.argtypes = [ct.c_void_p, ct.c_int]
+        L.orc_round_state_f32.argtypes = [ct.c_void_p]
         L.orc_run_until.argtypes = [ct.c_void_p, _D, ct.c_long]
         L.orc_run_until.restype = ct.c_long
         L.orc_time.argtypes = [ct.c_void_p]
@@ -213,6 +214,9 @@ class Oracle:
 
     def step(self, n=1):
         self._L.orc_step(self._s, n)
+
+    def round_state_f32(self):
+        self._L.orc_round_state_f32(self._s)
 
     def run_until(self, tmax, max_iters=1 << 62):
         return self._L.orc_run_until(self._s, tmax, max_iters)

@@ -720,6 +720,18 @@ void orc_iteration(orc_t *s)
     if (s->adaptive) s->dt2_next = orc_cfl_dt2(s, m_k);
 }
 
+/* test aid: round the state (h, um, vn and their "old" copies) to binary32
+ * and rebuild the derived fields, to measure what FP32 storage alone costs */
+void orc_round_state_f32(orc_t *s)
+{
+    for (int c = 0; c < s->NA; c++) {
+        s->h[c] = (double)(float)s->h[c];
+        s->um[c] = (double)(float)s->um[c];
+        s->vn[c] = (double)(float)s->vn[c];
+    }
+    orc_derived(s);
+}
+
 void orc_step(orc_t *s, int n)
 {
     for (int i = 0; i < n; i++) orc_iteration(s);

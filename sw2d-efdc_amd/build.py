@@ -18,7 +18,10 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libsw2d.so")
 DRIVER = os.path.join(HERE, "sw2d-efdc")
 
-HIP_SOURCES = ["sw2d_api.hip", "sw2d_fused.hip", "sw2d_plain.hip", "sw2d_host.cpp"]
+# (source, extra flags); the FP32 kernel may fuse multiply-adds, nothing else may
+HIP_SOURCES = [("sw2d_api.hip", []), ("sw2d_fused64.hip", []),
+               ("sw2d_fused32.hip", ["-ffp-contract=fast"]), ("sw2d_plain.hip", []),
+               ("sw2d_host.cpp", [])]
 DRIVER_SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp"]
 ARCH = os.environ.get("SW2D_OFFLOAD_ARCH", "gfx950")
 
@@ -38,9 +41,23 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 def build(verbose=False):
     """Compile every HIP/C++ source of the package (always recompiles)."""
     env = dict(os.environ)
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    procs = []
+    for src, extra in HIP_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, *extra, "-c", "-o", obj,
+               os.path.join(CSRC, src)]
+        if verbose:
+            print(" ".join(cmd))
+        procs.append((subprocess.Popen(cmd, env=env), cmd))
+        objs.append(obj)
+    for p, cmd in procs:
+        if p.wait() != 0:
+            raise subprocess.CalledProcessError(p.returncode, cmd)
     tmp = LIB + ".tmp"
-    cmd = [hipcc(), f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", tmp, *srcs]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True, env=env)

@@ -154,3 +154,245 @@ def apply(case_obj, spec):
         if case_obj.__class__.__name__ == "Oracle":
             case_obj.set_adaptive(spec.courant, 2.0 * spec.dt)
     return case_obj
+
+
+# ---------------------------------------------------------------------------
+# db/<case>/ tree in the reference's file layout (read by sw2d-efdc)
+# ---------------------------------------------------------------------------
+_SETUP1 = """#*******************************************************#
+# 1-SETUP: SW2D-GPU                                     #
+#*******************************************************#
+# START TIME: DAY, HOUR, MINUTES, SECONDS #
+
+{t0}
+
+# END TIME: DAY, HOUR, MINUTES, SECONDS #
+
+{t1}
+
+# TIME STEP #
+
+{dt}
+
+# TIME INTERVAL TO SHOW TIME ON SCREEM (SECONDS) #
+
+{dpout}
+
+# TIME INTERVAL TO WRITE THE RESULTS (SECONDS) #
+
+{dkout}
+
+# TEMPORAL RESOLUTION OF FLOW INPUT DATA (SECONDS) #
+
+{dtoq}
+
+# EVAPORATION IN THE LAKE: 1 ACTIVE, 0 INACTIVE #
+
+{evap}
+
+# MINIMUM WATER DEPTH FOR EVAPORATION #
+
+{ev_wl_min}
+
+# PERCENTAGE OF WATER LOST BY INTERCEPTION #
+
+{INT}
+
+# PERCENTAGE OF WATER LOST BY INFILTRATION #
+
+{INF}
+
+# PERCENTAGE OF WATER LOSS IN THE LAKE #
+
+{LWL}
+
+# MANNING COEFFICIENT #
+
+{manning}
+
+# OUTPUT X VELOCITY: 1 ACTIVE, 0 INACTIVE #
+
+{ovx}
+
+# OUTPUT Y VELOCITY: 1 ACTIVE, 0 INACTIVE #
+
+{ovy}
+
+# OUTPUT ELEVATIONS: 1 ACTIVE, 0 INACTIVE #
+
+{oel}
+
+# OUTPUT WATER DEPTH: 1 ACTIVE, 0 INACTIVE #
+
+{odp}
+
+# OUTPUT WATER DEPTH TIME SERIES: 1 ACTIVE, 0 INACTIVE #
+
+{oot}
+
+#*******************************************************#
+# 2-SETUP: EFDC-MPI                                     #
+#*******************************************************#
+# 3D HYDRODYNAMIC MODEL: 1 ACTIVE, 0 INACTIVE #
+
+{efdc}
+"""
+
+# the EFDC block: (comment lines, value) in file order
+_EFDC = [
+    (["#-------------- TEMPORAL CONTROLLERS -------------------#", "# EFDC BASE TIME #"], "basetime"),
+    (["# TIME ORIGIN OF RUN #"], 0.0),
+    (["# NUMBER OF TIME STEPS PER REFERENCE TIME PERIOD #"], 8640),
+    (["#------- ROUGHNESS AND DEPTH PARAMETERS ----------#", "# NUMBER OF VERTICAL LAYERS #"], "nlayers"),
+    (["# LOG BDRY LAYER CONST OR VARIABLE ROUGH HEIGHT ADJ IN METERS #"], 0.02),
+    (["# LOG BDRY LAYER VARIABLE ROUGHNESS HEIGHT CONVERT TO METERS #"], 1.0),
+    (["# MINIMUM DEPTH OF INPUTS DEPTHS IN METERS #"], 0.1),
+    (["# ADJUCTMENT TO DEPTH FIELD IN METERS #"], 0.0),
+    (["# CONVERTS INPUT DEPTH FIELD TO METERS #"], 1.0),
+    (["# DEPTH AT WHICH CELL OR FLOW FACE BECOMES DRY #"], 0.02),
+    (["# DEPTH AT WHICH CELL OR FLOW FACE BECOMES WET #"], 0.05),
+    (["# ADJUCTMENT TO BOTTOM BED ELEVATION FIELD IN METERS #"], 0.0),
+    (["# CONVERTS INPUT BOTTOM BED ELEVATION FIELD TO METERS #"], 1.0),
+    (["#----------------- ATMOSPHERIC FORCING -----------------#",
+      "# NUMBER OF WIND TIME SERIES (0 SETS WIND TO ZERO) #"], 0),
+    (["# NUMBER OF ATMOSPHERIC CONDITION TIME SERIES (0 SETS ALL  ZERO) #"], 0),
+    (["#---------- ATMOSPHERIC AND THERMAL PROPERTIES ---------#",
+      "# 0 DISTRIBUTE SW SOL RAD OVER WATER COL AND INTO BED, = 1 ALL TO SURF LAYER #"], 1),
+    (["# 1000 * EVAPORATIVE TRANSFER COEF, REVC<0 USE WIND SPD DEPD DRAG COEF #"], 1.5),
+    (["# 1000 * CONVECTIVE HEAT TRANSFER COEF, REVC<0 USE WIND SPD DEPD DRAG COEF #"], 1.5),
+    (["# FAST SCALE SOLAR SW RADIATION ATTENUATION COEFFCIENT 1./METERS #"], 2.0),
+    (["# SLOW SCALE SOLAR SW RADIATION ATTENUATION COEFFCIENT 1./METERS #"], 0.5),
+    (["# FRACTION OF SOLSR SW RADIATION ATTENUATED FAST  0<FSWRATF<1 #"], 0.5),
+    (["# DEPTH OR THICKNESS OF ACTIVE BED TEMPERATURE LAYER, METERS #"], 1.0),
+    (["# INITIAL BED TEMPERATURE #"], 20.0),
+    (["# CONVECTIVE HT COEFFCIENT BETWEEN BED AND BOTTOM WATER LAYER  NO DIM #"], 0.0),
+    (["# HEAT TRANS COEFFCIENT BETWEEN BED AND BOTTOM WATER LAYER  M/SEC #"], 0.0),
+    (["#----- CONTROLS FOR HORIZONTAL SURFACE ELEVATION OR PRESSURE CONTOURING -----#",
+      "# SURFACE ELEVATION OUTPUT #",
+      "# 1 TO WRITE FILE FOR SURFACE ELEVATION OR PRESSURE CONTOURING #",
+      "# 2 WRITE ONLY DURING LAST REFERENCE TIME PERIOD #"], 0),
+    (["# NUMBER OF WRITES PER REFERENCE TIME PERIOD #"], 1),
+    (["# 1 TO WRITE FILE FOR RESIDUAL SURFACE ELEVATION  CONTOURNG IN HORIZONTAL PLANE #"], 0),
+    (["# SURFACE ELEVATION WRITING FORMAT #",
+      "# 0 DOES NOT WRITE I,J,X,Y IN surfplt.out and rsurfplt.out FILES #",
+      "# 1 WRITES I,J ONLY IN surfplt.out and rsurfplt.out FILES #",
+      "# 2 WRITES I,J,X,Y  IN surfplt.out and rsurfplt.out FILES #",
+      "# 3 WRITES EFDC EXPLORER BINARY FORMAT FILES #"], 0),
+    (["#------ CONTROLS FOR HORIZONTAL PLANE VELOCITY VECTOR PLOTTING ----------#",
+      "# HORIZONTAL PLANE VELOCITY OUTPUT #",
+      "# 1 TO WRITE FILE FOR VELOCITY PLOTTING IN HORIZONTAL PLANE #",
+      "# 2 WRITE ONLY DURING LAST REFERENCE TIME PERIOD #"], 0),
+    (["# NUMBER OF WRITES PER REFERENCE TIME PERIOD #"], 1),
+    (["# 1 TO WRITE FILE FOR RESIDUAL VELOCITY PLOTTIN IN HORIZONTAL PLANE #"], 0),
+    (["# HORIZONTAL VELOCITY WRITING FORMAT #",
+      "# 0 DOES NOT WRITE I,J,X,Y IN velplth.out and rvelplth.out FILES #",
+      "# 1 WRITES I,J ONLY IN velplth.out and rvelplth.out FILES #",
+      "# 2 WRITES I,J,X,Y  IN velplth.out and rvelplth.out FILES #",
+      "# 3 WRITES EFDC EXPLORER BINARY FORMAT FILES #"], 0),
+    (["#--------------- CONTROLS FOR WRITING TO TIME SERIES FILES --------------#",
+      "# WRITE SCENARIO FOR THIS LOCATION #"], 1),
+    (["# 1 FOR TIME SERIES OF SURFACE ELEVATION #"], 1),
+    (["# 1 FOR TIME SERIES OF TRANSPORTED CONCENTRATION VARIABLES #"], 0),
+    (["# 1 FOR TIME SERIES OF EDDY VISCOSITY AND DIFFUSIVITY #"], 0),
+    (["# 1 FOR TIME SERIES OF EXTERNAL MODE HORIZONTAL VELOCITY #"], 0),
+    (["# 1 FOR TIME SERIES OF EXTERNAL MODE HORIZONTAL TRANSPORT #"], 0),
+    (["# 1 FOR TIME SERIES OF HORIZONTAL VELOCITY IN EVERY LAYER #"], 0),
+    (["#---------------------- SHOW.INP PARAMETERS ---------------------------#", "# NSTYPE #"], 1),
+    (["# NSHOWR #"], 1), (["# ISHOWC #"], 1), (["# JSHOWC #"], 1), (["# NSHFREQ #"], 1),
+    (["# ZSSMIN #"], 0), (["# ZSSMAX #"], 1), (["# SSALMAX #"], 0),
+    (["#----------------- DOMAIN DECOMPOSITION PARAMETERS --------------------#",
+      "# NUMBER OF PROCESSORS #"], 1),
+    (["# LOCALLY OPTIMAL RECTILINEAR PARTITION (LORP) TYPE #", "# 1 TYPE h #", "# 2 TYPE v #",
+      "# 3 TYPE r #"], 1),
+    (["#---------------- CHOOSE TO INCLUDE IN THE SIMULATION -----------------#",
+      "# DYE SIMULATION: 1 ACTIVE, 0 INACTIVE #"], 0),
+    (["# TEMPERATURE SIMULATION: 1 ACTIVE, 0 INACTIVE #"], 0),
+    (["# SALINITY SIMULATION: 1 ACTIVE, 0 INACTIVE #"], 0),
+    (["#----------------------- INITIAL CONDITIONS ---------------------------#",
+      "# INITIAL DYE CONCENTRATION IN THE LAKE #"], 0.0),
+    (["# INITIAL SALINITY IN THE LAKE #"], 0.0),
+    (["# WATER TEMPERATURE AT THE BOTTOM OF THE LAKE #"], 20.0),
+    (["# WATER TEMPERATURE ON THE LAKE SURFACE #"], 22.0),
+    (["#---- CATCHMENT-LAKE BOUNDARIES (WATER INPUTS FROM THE CATCHMENT) -----#",
+      "# DYE CONCENTRATION #"], 0.0),
+    (["# SALINITY #"], 0.0),
+]
+
+
+def _hms(t):
+    d, r = divmod(int(round(t)), 86400)
+    h, r = divmod(r, 3600)
+    m, s = divmod(r, 60)
+    return f"{d} {h} {m} {s}"
+
+
+def _esri(path, a, cellsize, nodata, fmt, xll=0.0, yll=0.0):
+    R, C = a.shape
+    with open(path, "w") as f:
+        f.write(f"ncols         {C}\nnrows         {R}\nxllcorner     {float(xll)!r}\n"
+                f"yllcorner     {float(yll)!r}\ncellsize      {float(cellsize)!r}\n"
+                f"NODATA_value  {float(nodata)!r}\n")
+        for row in a:
+            f.write(" ".join(fmt(v) for v in row) + "\n")
+
+
+def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=None,
+             efdc=False, basetime=86400.0, nlayers=1, evaporation=None, INT=0.0, INF=0.0,
+             LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1)):
+    """Write db/dir.txt and db/<name>/ for a fixed-step spec in the reference's
+    layout.  probes: (x, y) metres for coord_out.dat; sources: (xs, ys, Q, dtoq);
+    evaporation: (albedo, T[], Rg[]) with one value per rain interval."""
+    import os
+    base = os.path.join(root, "db", name)
+    for d in ("SW2D/input", "SW2D/output", "EFDC/input", "EFDC/output"):
+        os.makedirs(os.path.join(base, d), exist_ok=True)
+    with open(os.path.join(root, "db", "dir.txt"), "w") as f:
+        f.write(f"dir_number: 1\n{name}\n")
+    dtoq = sources[3] if sources else 3600.0
+    ovx, ovy, oel, odp, oot = outputs
+    txt = _SETUP1.format(t0="0 0 0 0", t1=_hms(spec.t_end), dt=repr(spec.dt), dpout=repr(dpout),
+                         dkout=repr(dkout), dtoq=repr(dtoq),
+                         evap=1 if evaporation else 0, ev_wl_min=repr(EV_WL_min), INT=repr(INT),
+                         INF=repr(INF), LWL=repr(LWL), manning=repr(spec.manning), ovx=ovx,
+                         ovy=ovy, oel=oel, odp=odp, oot=oot if probes else 0,
+                         efdc=1 if efdc else 0)
+    if efdc:
+        lines = []
+        for comments, v in _EFDC:
+            lines += comments
+            val = {"basetime": basetime, "nlayers": nlayers}.get(v, v) if isinstance(v, str) else v
+            lines += ["", repr(float(val)) if isinstance(val, float) else str(val), ""]
+        txt += "\n" + "\n".join(lines) + "\n"
+    with open(os.path.join(base, "setup.dat"), "w") as f:
+        f.write(txt)
+    inp = os.path.join(base, "SW2D", "input")
+    _esri(os.path.join(inp, "dem.dat"), spec.z, spec.cellsize, spec.nodata,
+          lambda v: repr(float(v)))
+    with open(os.path.join(inp, "initial_condition.dat"), "w") as f:
+        for row in spec.h0:
+            f.write(" ".join(repr(float(v)) for v in row) + "\n")
+    xs, ys, Q = (sources[0], sources[1], np.asarray(sources[2]).reshape(-1, len(sources[0]))) \
+        if sources else ([], [], np.zeros((1, 0)))
+    with open(os.path.join(inp, "coord_source_sink.dat"), "w") as f:
+        f.write(f"nst {len(xs)}\n" + "".join(f"{x!r} {y!r}\n" for x, y in zip(xs, ys)))
+    with open(os.path.join(inp, "Q_source_sink.dat"), "w") as f:
+        f.write("flow (m3/s)\n" + "\n".join(" ".join(repr(float(v)) for v in row) for row in Q))
+    rain = spec.rain if spec.rain is not None else np.zeros(int(spec.t_end // spec.dtrain) + 2)
+    with open(os.path.join(inp, "rain.dat"), "w") as f:
+        # no newline after the last row: the reader takes (newlines - 1) values
+        f.write(f"{spec.dtrain!r}\nTime (hour) Rainfall (mm)\n" +
+                "\n".join(f"{i} {float(v)!r}" for i, v in enumerate(rain)))
+    if probes:
+        with open(os.path.join(inp, "coord_out.dat"), "w") as f:
+            f.write(f"n_out {len(probes)}\n" + "".join(f"{x!r} {y!r}\n" for x, y in probes))
+    if evaporation:
+        albedo, T, Rg = evaporation
+        with open(os.path.join(inp, "temperatures.dat"), "w") as f:
+            f.write("Temperature\n" + "\n".join(repr(float(v)) for v in T) + "\n")
+        with open(os.path.join(inp, "solar_radiation.dat"), "w") as f:
+            f.write(f"albedo: {albedo!r}\nRadiation\n" + "\n".join(repr(float(v)) for v in Rg) + "\n")
+    if efdc:
+        mask = spec.lake_mask if spec.lake_mask is not None else np.zeros(spec.z.shape, np.int32)
+        _esri(os.path.join(inp, "mask.dat"), mask.astype(np.int64), spec.cellsize, -9999,
+              lambda v: str(int(v)))
+    return base

@@ -98,15 +98,30 @@ StepCommon make_common(sw2d_handle *h)
 
 bool fused(const sw2d_handle *h) { return h->prm.kernel == SW2D_KERNEL_FUSED; }
 
+// Default tile height of the fused kernel (every tile is one wave): 16 rows,
+// lowered towards 8 while the grid has fewer tiles than resident wave slots
+// (occupancy x CUs).  sw2d_autotune replaces it by a measured choice.
 int choose_tile_rows(const sw2d_handle *h)
 {
     if (h->prm.tile_rows > 0) return std::min(h->prm.tile_rows, 512);
-    // enough tiles to give every CU several waves, tiles tall enough that the
-    // 6 warm-up rows stay a small overhead
+    int per_cu = h->esize == 8 ? fused_waves_per_cu_f64(h->prm.adaptive)
+                               : fused_waves_per_cu_f32(h->prm.adaptive);
+    hipDeviceProp_t prop;
+    int cus = 0;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) cus = prop.multiProcessorCount;
+    const long slots = (long)std::max(per_cu, 1) * std::max(cus, 1);
     const int nstrips = (h->C + 57) / 58;
-    int H = 128;
-    while (H > 32 && (long)nstrips * ((h->R + H - 1) / H) < 4096) H /= 2;
+    int H = 16;
+    while (H > 8 && (long)nstrips * ((h->R + H - 1) / H) < slots) H--;
     return H;
+}
+
+void set_tiles(sw2d_handle *h, int H)
+{
+    h->H = H;
+    h->nstrips = (h->C + h->W - 1) / h->W;
+    h->nchunks = (h->R + H - 1) / H;
+    h->ntiles = h->nstrips * h->nchunks;
 }
 
 int upload_source_cells(sw2d_handle *h, const int *rr, const int *cc, int n)
@@ -196,7 +211,7 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
     const size_t RC = (size_t)R * C;
     const double nodata = grid->nodata;
 
-    // host copy of z (outputs print it) and initial volume
+    // initial volume
     double v0 = 0.0;
     for (size_t i = 0; i < RC; i++)
         if (z[i] != nodata) v0 += h0[i];
@@ -206,37 +221,32 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
         h->P = round_up(C, 16);
         const size_t n = (size_t)R * h->P;
         h->W = 58;
-        h->H = choose_tile_rows(h);
-        h->nstrips = (C + h->W - 1) / h->W;
-        h->nchunks = (R + h->H - 1) / h->H;
-        h->ntiles = h->nstrips * h->nchunks;
+        set_tiles(h, choose_tile_rows(h));
         h->nparts = h->ntiles;
-        void **bufs[] = {&h->h[0], &h->h[1], &h->um[0], &h->um[1], &h->vn[0], &h->vn[1], &h->z};
+        void **bufs[] = {&h->h[0], &h->h[1], &h->um[0], &h->um[1], &h->vn[0], &h->vn[1]};
         for (void **b : bufs) {
             if (h->esize == 8) { if ((rc = dalloc(h, (double **)b, n))) return bail(rc); }
             else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
         }
+        if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
         if ((rc = dalloc(h, &h->scratch, RC))) return bail(rc);
-        // pitched host images: inactive cells carry NaN in z
+        // pitched images: inactive cells carry NaN in z, which stays FP64
+        std::vector<double> zi(n, 0.0);
+        for (int r = 0; r < R; r++)
+            for (int j = 0; j < C; j++) {
+                const double zv = z[(size_t)r * C + j];
+                zi[(size_t)r * h->P + j] = (zv != nodata) ? zv : NAN;
+            }
+        e = hipMemcpy(h->z, zi.data(), n * 8, hipMemcpyHostToDevice);
         if (h->esize == 8) {
-            std::vector<double> zi(n, 0.0), hi(n, 0.0);
+            std::vector<double> hi(n, 0.0);
             for (int r = 0; r < R; r++)
-                for (int j = 0; j < C; j++) {
-                    const double zv = z[(size_t)r * C + j];
-                    zi[(size_t)r * h->P + j] = (zv != nodata) ? zv : NAN;
-                    hi[(size_t)r * h->P + j] = h0[(size_t)r * C + j];
-                }
-            e = hipMemcpy(h->z, zi.data(), n * 8, hipMemcpyHostToDevice);
+                for (int j = 0; j < C; j++) hi[(size_t)r * h->P + j] = h0[(size_t)r * C + j];
             if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 8, hipMemcpyHostToDevice);
         } else {
-            std::vector<float> zi(n, 0.0f), hi(n, 0.0f);
+            std::vector<float> hi(n, 0.0f);
             for (int r = 0; r < R; r++)
-                for (int j = 0; j < C; j++) {
-                    const double zv = z[(size_t)r * C + j];
-                    zi[(size_t)r * h->P + j] = (zv != nodata) ? (float)zv : NAN;
-                    hi[(size_t)r * h->P + j] = (float)h0[(size_t)r * C + j];
-                }
-            e = hipMemcpy(h->z, zi.data(), n * 4, hipMemcpyHostToDevice);
+                for (int j = 0; j < C; j++) hi[(size_t)r * h->P + j] = (float)h0[(size_t)r * C + j];
             if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 4, hipMemcpyHostToDevice);
         }
         if (e != hipSuccess) { hip_fail(h, e, "upload"); return bail(SW2D_E_HIP); }
@@ -272,7 +282,7 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
     h->host_mstep = 0;
     h->step_index = 0;
 
-    e = fused(h) ? launch_init(h) : launch_plain_init(h);
+    e = !fused(h) ? launch_plain_init(h) : (h->esize == 8 ? launch_init_f64(h) : launch_init_f32(h));
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) { hip_fail(h, e, "init kernel"); return bail(SW2D_E_HIP); }
     *out = h;
@@ -462,6 +472,75 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask)
     return SW2D_OK;
 }
 
+int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (iters < 1) return fail(h, SW2D_E_ARG, "iters must be positive");
+    if (h->step_index != 0) return fail(h, SW2D_E_STATE, "autotune runs before the first iteration");
+    if (!fused(h)) {
+        if (tile_rows) *tile_rows = 0;
+        return SW2D_OK;
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    const int cand[] = {8, 10, 12, 14, 16, 18, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+    // scratch bookkeeping: trials read the real state and write only the spare
+    // buffer, a copy of the clock ring, scratch partials/ledger/inflow sums
+    const int max_tiles = ((h->C + h->W - 1) / h->W) * ((h->R + 7) / 8);
+    Clock *clk_s = nullptr;
+    Partial *part_s = nullptr;
+    Ledger *led_s = nullptr;
+    double *bc_s = nullptr;
+    int rc;
+    if ((rc = dalloc(h, &clk_s, 3)) || (rc = dalloc(h, &part_s, 2 * (size_t)max_tiles)) ||
+        (rc = dalloc(h, &led_s, 1)) || (rc = dalloc(h, &bc_s, std::max(h->nbc, 1))))
+        return rc;
+    hipEvent_t e0, e1;
+    HIPCHK(h, hipEventCreate(&e0));
+    HIPCHK(h, hipEventCreate(&e1));
+    const int H0 = h->H;
+    int best = H0;
+    float best_ms = 1e30f;
+    for (int H : cand) {
+        if (H > h->R && H != cand[0]) break;
+        set_tiles(h, H);
+        StepCommon sc = make_common(h);
+        HIPCHK(h, hipMemcpyAsync(clk_s, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, h->stream));
+        sc.clk = clk_s;
+        sc.part_out = part_s;
+        sc.part_in = part_s + max_tiles;
+        sc.nparts = h->ntiles;
+        sc.ledger = led_s;
+        sc.hydro = nullptr;
+        sc.bc_acc = bc_s;
+        auto launch = [&]() {
+            return h->esize == 8 ? launch_fused_f64(h, sc) : launch_fused_f32(h, sc);
+        };
+        hipError_t e = launch();  // warm-up of this shape
+        if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
+        HIPCHK(h, hipEventRecord(e0, h->stream));
+        for (int i = 0; i < iters; i++)
+            if ((e = launch()) != hipSuccess) return hip_fail(h, e, "autotune launch");
+        HIPCHK(h, hipEventRecord(e1, h->stream));
+        HIPCHK(h, hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = H; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s);
+    set_tiles(h, best);
+    if (h->ntiles > h->nparts) {
+        dfree(h, h->part);
+        h->part = nullptr;
+        if ((rc = dalloc(h, &h->part, 2 * (size_t)h->ntiles))) return rc;
+    }
+    h->nparts = h->ntiles;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (tile_rows) *tile_rows = best;
+    return SW2D_OK;
+}
+
 int sw2d_step(sw2d_handle *h, int n_iters)
 {
     if (check_idle(h)) return SW2D_E_ARG;
@@ -474,7 +553,8 @@ int sw2d_step(sw2d_handle *h, int n_iters)
             HIPCHK(h, hipEventCreate(&e1));
             HIPCHK(h, hipEventRecord(e0, h->stream));
         }
-        hipError_t e = fused(h) ? launch_fused(h, sc) : launch_plain(h, sc);
+        hipError_t e = !fused(h) ? launch_plain(h, sc)
+                       : (h->esize == 8 ? launch_fused_f64(h, sc) : launch_fused_f32(h, sc));
         if (e != hipSuccess) return hip_fail(h, e, "step launch");
         if (h->timing) {
             HIPCHK(h, hipEventRecord(e1, h->stream));
@@ -579,7 +659,8 @@ int sw2d_get_field(sw2d_handle *h, int field, double *out)
     const size_t RC = (size_t)h->R * h->C;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (fused(h)) {
-        hipError_t e = launch_derive(h, field, h->scratch);
+        hipError_t e = h->esize == 8 ? launch_derive_f64(h, field, h->scratch)
+                                     : launch_derive_f32(h, field, h->scratch);
         if (e != hipSuccess) return hip_fail(h, e, "derive");
         HIPCHK(h, hipMemcpyAsync(out, h->scratch, RC * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -690,7 +771,7 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
     if (check_idle(h) || !bytes) return SW2D_E_ARG;
     long long b = 0;
     const size_t RC = (size_t)h->R * h->C;
-    if (fused(h)) b += (long long)7 * h->R * h->P * h->esize + (long long)RC * 8 + (h->cls ? (long long)h->R * h->P : 0);
+    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (long long)RC * 8 + (h->cls ? (long long)h->R * h->P : 0);
     else b += (long long)16 * h->NA * 8;
     b += (long long)2 * h->nparts * sizeof(Partial);
     *bytes = b;

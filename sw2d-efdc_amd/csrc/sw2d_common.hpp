@@ -70,6 +70,14 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+__device__ __forceinline__ double wave_max(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    return v;
+}
 __device__ __forceinline__ double wave_min(double v)
 {
     for (int off = 32; off >= 1; off >>= 1) {

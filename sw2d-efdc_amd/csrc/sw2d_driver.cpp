@@ -1,0 +1,283 @@
+// sw2d_driver.cpp -- sw2d-efdc: the drop-in executable for the reference's
+// SW2D run (main() of SW2D-EFDC.cu, 3682-5166) on the MI355X library.
+//
+// For every case of db/dir.txt it reads the case files (3733-4460), runs the
+// reference's time loop (4789-4877) through the C ABI of libsw2d, and writes
+// the reference's outputs at the reference's output steps (4880-5072):
+// SW2D/output/Results_<k>.vtk, SW2D/output/WaterLevel.txt,
+// SW2D/output/TimeSimu_<k>.txt and, for coupled cases, the lake-inflow series
+// EFDC/input/QSER.INP (couple_BC_efdc + write_efdc_QSER, with the
+// reference's one-output lag of the velocities).  The EFDC side (EFDC.INP
+// card patching, the other EFDC input files, GEFDC/Gorp and the mpirun of the
+// Fortran lake model) is not part of this build.
+//
+// usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32]
+//                  [--device N] [--autotune] [--parse-only]
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <ctime>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/sw2d.h"
+#include "sw2d_io.hpp"
+
+namespace {
+
+struct Options {
+    std::string root = ".";
+    int kernel = SW2D_KERNEL_FUSED;
+    int precision = SW2D_FP64;
+    int device = 0;
+    bool autotune = false;
+    bool parse_only = false;
+};
+
+bool parse_args(int argc, char **argv, Options &o)
+{
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        auto next = [&](std::string &v) {
+            if (i + 1 >= argc) return false;
+            v = argv[++i];
+            return true;
+        };
+        std::string v;
+        if (a == "--root" && next(v)) o.root = v;
+        else if (a == "--kernel" && next(v)) o.kernel = (v == "plain") ? SW2D_KERNEL_PLAIN : SW2D_KERNEL_FUSED;
+        else if (a == "--precision" && next(v)) o.precision = (v == "fp32") ? SW2D_FP32 : SW2D_FP64;
+        else if (a == "--device" && next(v)) o.device = atoi(v.c_str());
+        else if (a == "--autotune") o.autotune = true;
+        else if (a == "--parse-only") o.parse_only = true;
+        else {
+            fprintf(stderr, "unknown argument %s\n", a.c_str());
+            return false;
+        }
+    }
+    return true;
+}
+
+// std::ostream formatting of round(x), as the reference builds file names
+std::string tempo_name(double x)
+{
+    std::stringstream out;
+    out << x;
+    return out.str();
+}
+
+void print_parse_summary(const sw2dio::Case &c, double time0, double timmax)
+{
+    const auto &s = c.setup;
+    printf("{\"dir\": \"%s\", \"rows\": %d, \"cols\": %d, \"cellsize\": %.17g, \"nodata\": %.17g, "
+           "\"xll\": %.17g, \"yll\": %.17g, \"dt\": %.17g, \"dpout\": %.17g, \"dkout\": %.17g, "
+           "\"dtoq\": %.17g, \"manning\": %.17g, \"INT\": %.17g, \"INF\": %.17g, \"LWL\": %.17g, "
+           "\"EV_WL_min\": %.17g, \"evaporation\": %d, \"outputs\": [%d, %d, %d, %d, %d], "
+           "\"efdc\": %d, \"efdc_basetime\": %.17g, \"nlayers\": %d, \"time0\": %.17g, "
+           "\"timmax\": %.17g, \"nst\": %zu, \"nq\": %d, \"dtrain\": %.17g, \"nrain\": %zu, "
+           "\"rain_sum\": %.17g, \"n_out\": %zu, \"z_sum\": %.17g, \"h0_sum\": %.17g, "
+           "\"mask_lake\": %ld}\n",
+           c.dir.c_str(), c.dem.rows, c.dem.cols, c.dem.cellsize, c.dem.nodata, c.dem.xll, c.dem.yll,
+           s.dt, s.dpout, s.dkout, s.dtoq, s.manning, s.INT, s.INF, s.LWL, s.EV_WL_min,
+           s.evaporation_on, s.out_vx, s.out_vy, s.out_elev, s.out_depth, s.out_outlet, s.efdc,
+           s.efdc_basetime, s.nlayers, time0, timmax, c.src_x.size(), c.nq, c.dtrain, c.rain.size(),
+           [&] { double a = 0; for (double r : c.rain) a += r; return a; }(), c.out_x.size(),
+           [&] { double a = 0; for (double z : c.dem.v) if (z != c.dem.nodata) a += z; return a; }(),
+           [&] { double a = 0; for (double h : c.h0) a += h; return a; }(),
+           [&] { long n = 0; for (int m : c.mask) n += (m == 1); return n; }());
+}
+
+int run_case(const Options &opt, const std::string &name)
+{
+    printf("%s\n", " ******************************************************************** ");
+    printf("CASE: %s\n", name.c_str());
+    sw2dio::Case c;
+    const std::string err = sw2dio::read_case(opt.root, name, c);
+    if (!err.empty()) {
+        printf("%s\n", err.c_str());
+        return 1;
+    }
+    const sw2dio::Setup &s = c.setup;
+    const int R = c.dem.rows, C = c.dem.cols;
+    const double dt = s.dt;
+    // output cadence and time window (4187-4188, 4230-4235)
+    long lpout = lround(s.dpout / dt), lkout = lround(s.dkout / dt);
+    if (lpout <= 0) lpout = 1;
+    if (lkout <= 0) lkout = 1;
+    const double time0 = 3600.0 * 24.0 * s.tday0 + 3600.0 * s.thour0 + 60.0 * s.tmin0 + s.tsec0;
+    double timmax = 3600.0 * 24.0 * s.tday + 3600.0 * s.thour + 60.0 * s.tmin + s.tsec;
+    if ((s.tday == 0) & (s.thour == 0) & (s.tmin == 0) & (s.tsec == 0))
+        timmax = ((double)c.rain.size() - 1) * c.dtrain;  // (cont_rain - 2) * dtrain
+    if (opt.parse_only) {
+        print_parse_summary(c, time0, timmax);
+        return 0;
+    }
+
+    std::vector<double> z = c.dem.v;
+    if (s.efdc == 1) sw2d_lake_catchment_smooth(R, C, c.mask.data(), z.data());  // 4460
+
+    sw2d_grid_desc grid{R, C, c.dem.xll, c.dem.yll, c.dem.cellsize, c.dem.nodata};
+    sw2d_params prm;
+    sw2d_default_params(&prm);
+    prm.g = 9.80;
+    prm.manning = s.manning;
+    prm.th = 1.0e-4;
+    prm.dt = dt;
+    prm.time0 = time0;
+    prm.INT = s.INT; prm.INF = s.INF; prm.LWL = s.LWL; prm.EV_WL_min = s.EV_WL_min;
+    prm.kernel = opt.kernel;
+    prm.precision = opt.precision;
+    prm.device = opt.device;
+    sw2d_handle *h = nullptr;
+    if (sw2d_create(&grid, z.data(), c.h0.data(), &prm, &h) != SW2D_OK) {
+        printf("sw2d_create failed: %s\n", sw2d_last_error(nullptr));
+        return 1;
+    }
+    auto check = [&](int rc, const char *what) {
+        if (rc != SW2D_OK) {
+            printf("%s failed: %s\n", what, sw2d_last_error(h));
+            return false;
+        }
+        return true;
+    };
+    bool ok = check(sw2d_set_rain_series(h, c.rain.data(), (int)c.rain.size(), c.dtrain), "rain");
+    ok = ok && check(sw2d_set_sources(h, c.src_x.data(), c.src_y.data(), (int)c.src_x.size(),
+                                      c.Q.data(), c.nq, s.dtoq), "sources");
+    if (ok && s.evaporation_on == 1) {
+        std::vector<double> ev(c.T.size());
+        sw2d_evaporation_series(c.albedo, c.T.data(), c.Rg.data(), (int)c.T.size(), c.dtrain, ev.data());
+        ok = check(sw2d_set_evaporation(h, ev.data(), (int)ev.size()), "evaporation");
+    }
+    if (ok && opt.autotune) {
+        int H = 0;
+        ok = check(sw2d_autotune(h, 10, &H), "autotune");
+    }
+    if (!ok) {
+        sw2d_destroy(h);
+        return 1;
+    }
+
+    const std::string outdir = c.dir + "/SW2D/output/";
+    FILE *wl = fopen((outdir + "WaterLevel.txt").c_str(), "w");
+    if (!wl) {
+        printf("cannot write %sWaterLevel.txt\n", outdir.c_str());
+        sw2d_destroy(h);
+        return 1;
+    }
+    if (s.out_outlet != 1) fprintf(wl, "%s\n", "Outlet off");
+    const double res = c.dem.cellsize;
+    std::vector<double> outx(c.out_x.size()), outy(c.out_y.size());
+    for (size_t i = 0; i < outx.size(); i++) {
+        outx[i] = c.out_x[i] / res;
+        outy[i] = c.out_y[i] / res;
+    }
+
+    // coupling state (4241-4403): records, their values per output, lagged velocities
+    const size_t RC = (size_t)R * C;
+    sw2dio::QserSeries qser;
+    std::vector<double> uua_host(RC, 0.0), vva_host(RC, 0.0);
+    const int cap = (int)RC;
+    std::vector<int> cells, I, J, counts(4);
+    std::vector<double> q;
+    if (s.efdc == 1) {
+        cells.resize(4 * RC); I.resize(4 * RC); J.resize(4 * RC); q.resize(4 * RC);
+    }
+    bool efdc_note = false;
+
+    std::vector<double> hbuf(RC), f1(RC), f2(RC);
+    double time = time0;
+    long mstep = 0;
+    int tq = -1;
+    long out0 = -1;
+    long pending = 0;
+    std::string tempo = "0";
+    const std::clock_t start = std::clock();
+    while (time + dt <= timmax) {
+        out0 = out0 + 1;
+        if (mstep % lpout == 0) printf(" %d\n", (int)time);
+        pending++;
+        time = time + dt;
+        mstep = mstep + 1;
+        time = time + dt;
+        mstep = mstep + 1;
+        if (!((mstep % lkout == 0) || (out0 == 0))) continue;
+        if (!check(sw2d_step(h, (int)pending), "step")) break;
+        pending = 0;
+        tq = tq + 1;
+        tempo = (out0 > 0) ? tempo_name(round(mstep * dt / s.dkout)) : std::string("0");
+        if (!check(sw2d_get_field(h, SW2D_FIELD_H, hbuf.data()), "get_field")) break;
+        if (s.efdc == 1) {
+            const double efdc_time = mstep * dt / s.efdc_basetime;
+            const double *hh = (tq == 0) ? c.h0.data() : hbuf.data();
+            const double tt = (tq == 0) ? 0.0000 : efdc_time;
+            sw2d_couple_bc(R, C, res, res, c.mask.data(), hh, vva_host.data(), uua_host.data(), cap,
+                           cells.data(), I.data(), J.data(), q.data(), counts.data());
+            // records come back S, N, E, W; QSER.INP lists N, S, E, W
+            static const int kind_of_edge[4] = {1, 0, 2, 3};
+            for (int e = 0; e < 4; e++) {
+                const int k = kind_of_edge[e];
+                qser.nrec[e] = counts[k];
+                qser.time[e].push_back(counts[k] > 0 ? tt : 0.0);
+                qser.val[e].emplace_back(q.begin() + (size_t)k * cap, q.begin() + (size_t)k * cap + counts[k]);
+            }
+            // the velocities used by the next output are copied after the
+            // coupling call (4913-4916)
+            sw2d_get_field(h, SW2D_FIELD_UUA, uua_host.data());
+            sw2d_get_field(h, SW2D_FIELD_VVA, vva_host.data());
+            sw2dio::write_qser(c.dir + "/EFDC/input/QSER.INP", 0, s.nlayers, qser, s.efdc_basetime, tq);
+            if (tq == 0 && !efdc_note) {
+                printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
+                efdc_note = true;
+            }
+        }
+        sw2dio::VtkFields vf;
+        const double *depth = (out0 == 0) ? c.h0.data() : hbuf.data();
+        if (s.out_depth == 1) {
+            vf.depth = depth;
+            if (s.out_outlet == 1)
+                sw2dio::write_water_level(wl, R, C, c.dem.xll / res, c.dem.yll / res, outx, outy, depth);
+        }
+        if (s.out_vx == 1) {
+            sw2d_get_field(h, SW2D_FIELD_VV, f1.data());
+            vf.vv = f1.data();
+        }
+        if (s.out_vy == 1) {
+            sw2d_get_field(h, SW2D_FIELD_UU, f2.data());
+            vf.uu = f2.data();
+        }
+        if (s.out_elev == 1) vf.z = z.data();
+        sw2dio::write_vtk(outdir + "Results_" + tempo + ".vtk", C, R, res, res, c.dem.xll, c.dem.yll, vf);
+    }
+    if (pending > 0) check(sw2d_step(h, (int)pending), "step");
+    sw2d_sync(h);
+    const double duration = (std::clock() - start) / (double)CLOCKS_PER_SEC;
+    printf("printf: %g\n", duration);
+    FILE *ts = fopen((outdir + "TimeSimu_" + tempo + ".txt").c_str(), "w");
+    if (ts) {
+        fprintf(ts, " %lf\n", duration);
+        fclose(ts);
+    }
+    fclose(wl);
+    if (s.efdc == 1) printf(" the EFDC lake model is not run by this build (QSER.INP is ready)\n");
+    sw2d_destroy(h);
+    return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv)
+{
+    Options opt;
+    if (!parse_args(argc, argv, opt)) return 2;
+    std::vector<std::string> names;
+    const std::string err = sw2dio::read_dir_list(opt.root, names);
+    if (!err.empty()) {
+        printf("%s\n", err.c_str());
+        return 1;
+    }
+    int rc = 0;
+    for (const std::string &n : names) rc |= run_case(opt, n);
+    return rc;
+}

@@ -1,0 +1,20 @@
+// sw2d_fused32.hip -- FP32 instantiation of the fused SW2D kernel (precision
+// form: FP32 state, fluxes and local arithmetic; FP64 bed elevation, water
+// surfaces, time, CFL minimum, volume ledger and hydrograph).  Built with
+// -ffp-contract=fast: FP32 multiply-adds are fused.
+#include "sw2d_fused_kernel.hpp"
+
+namespace sw2d {
+
+hipError_t launch_fused_f32(sw2d_handle *h, const StepCommon &sc) { return launch_fused_impl<float>(h, sc); }
+int fused_waves_per_cu_f32(int adaptive)
+{
+    int nb = 0;
+    const void *fn = adaptive ? (const void *)fused_step<float, true> : (const void *)fused_step<float, false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
+    return nb;
+}
+hipError_t launch_init_f32(sw2d_handle *h) { return launch_init_impl<float>(h); }
+hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out) { return launch_derive_impl<float>(h, field, out); }
+
+}  // namespace sw2d

@@ -1,0 +1,30 @@
+// sw2d_fused64.hip -- FP64 instantiation of the fused SW2D kernel (parity
+// mode).  Built with -ffp-contract=off: no multiply-add is fused, so each
+// expression rounds in the order it is written, as in the CPU FP64 check.
+#include "sw2d_fused_kernel.hpp"
+
+namespace sw2d {
+
+__global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *out)
+{
+    ledger_peek(s, out, threadIdx.x);
+}
+
+hipError_t launch_fused_f64(sw2d_handle *h, const StepCommon &sc) { return launch_fused_impl<double>(h, sc); }
+int fused_waves_per_cu_f64(int adaptive)
+{
+    int nb = 0;
+    const void *fn = adaptive ? (const void *)fused_step<double, true> : (const void *)fused_step<double, false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
+    return nb;
+}
+hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double>(h); }
+hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out) { return launch_derive_impl<double>(h, field, out); }
+
+hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc)
+{
+    hipLaunchKernelGGL(peek_kernel, dim3(1), dim3(64), 0, h->stream, sc, h->peek);
+    return hipGetLastError();
+}
+
+}  // namespace sw2d

@@ -1,12 +1,14 @@
-// sw2d_fused.hip -- one fused kernel per SW2D loop iteration.
+// sw2d_fused_kernel.hpp -- one fused kernel per SW2D loop iteration.
 //
 // The reference (SW2D-EFDC.cu:4798-4877) runs nine grid-stride kernels per
 // iteration over ~20 FP64 arrays: flux, continuity, treat_error, continuity,
 // hm_hn, uu1_vv1, uu_vv, uua_vva, forward.  The complete state between
 // iterations is only (h, um, vn) (bed z is constant); every other array is a
 // local function of it.  This kernel reads (h, um, vn, z) once and writes
-// (h, um, vn) once per iteration, so one iteration moves 7 arrays:
-// 56 B/cell in FP64, 28 B/cell in FP32.
+// (h, um, vn) once per iteration: 56 B/cell in FP64; 32 B/cell in the FP32
+// form, whose bed elevation stays FP64 (it is input data: rounding a DEM to
+// binary32 moves it by up to half an ulp of its relief, which alone shifts
+// depths by ~1 mm on a 2.5 m-relief test DEM, while FP32 state costs 0.2 mm).
 //
 // Work decomposition.  A tile is one wavefront (64 lanes) and covers 58
 // output columns x H output rows.  Lane l holds column j = j0 - 3 + l: the
@@ -20,12 +22,14 @@
 //   the second continuity of row k-3              (continuity at t + dt)
 // and stores row k-3 of the new state.  Values of a row are carried in
 // registers while they are needed (7-row window); east/west neighbours are
-// cross-lane shuffles.  No LDS staging and no barriers: each wave is
-// independent.  The grid is one wave per tile plus one bookkeeping wave.
+// DPP lane moves.  No LDS staging and no barriers: each wave is independent.
+// The grid is one wave per tile plus the bookkeeping waves.
 //
-// Every FP64 expression is written in the reference's evaluation order and
-// the file is compiled with -ffp-contract=off, so the FP64 results equal the
-// reference's arithmetic (as restated by the CPU FP64 check) bit for bit.
+// Instantiated twice: sw2d_fused64.hip (T = double, built with
+// -ffp-contract=off: every FP64 expression is written in the reference's
+// evaluation order, so the results equal the reference arithmetic as restated
+// by the CPU FP64 check, bit for bit) and sw2d_fused32.hip (T = float).
+#pragma once
 #include "sw2d_common.hpp"
 
 namespace sw2d {
@@ -75,10 +79,10 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     const int r1 = (r0 + a.H < R) ? r0 + a.H : R;
 
     const T th = (T)s.th, gg = (T)s.g, dx = (T)s.dx, dy = (T)s.dy;
+    const T rdx = (T)(1.0 / s.dx), rdy = (T)(1.0 / s.dy);
     const T dt2 = (T)st.dt2;
     const T dt2dx = (T)(st.dt2 / s.dx), dt2dy = (T)(st.dt2 / s.dy);
     const T rn = (T)s.rn;
-    const T hdt2 = (T)st.dt2;  // dt2 of the friction factor (dt2*ram*0.50)
 
     // forcing of this iteration: first continuity at t, second at t + dt
     const double rr1 = rain_rate(s, st.t), rr2 = rain_rate(s, st.tm);
@@ -105,7 +109,8 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     T qo_eff = 0;
 
     // carried rows: suffix 1 = row k-1, 2 = k-2, 3 = k-3
-    T h1 = 0, h2 = 0, h3 = 0, h1w = 0, z1 = 0, z2 = 0, z1w = 0;
+    T h1 = 0, h2 = 0, h3 = 0, h1w = 0;
+    double z1 = 0, z2 = 0, z1w = 0;
     bool a1 = false, a2 = false, a3 = false, a1w = false;
     T u1 = 0, u2 = 0, u1e = 0, u1w = 0, v1 = 0, v2 = 0, v1e = 0, v1w = 0;
     T uu1_1 = 0, uu1_2 = 0, uu1_1w = 0;
@@ -115,50 +120,52 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     T umnP = 0, vnnP = 0, umpP = 0, vnpP = 0;
     bool dryP = false;
 
-    double acc_vol = 0, acc_src = 0, acc_clip = 0, acc_q = 0, cfl = INFINITY;
+    double acc_vol = 0, acc_src = 0, acc_clip = 0, acc_q = 0;
+    // CFL: the largest |u| + sqrt(g h) per direction; min_f dx / s_f = dx / max_f s_f
+    // because rounding dx / s is monotone in s
+    double smax_x = 0.0, smax_y = 0.0;
 
-    // software pipeline: row k+1 is loaded while row k is processed
+    // software pipeline: row k+1 is loaded while row k is processed.  Rows
+    // outside the grid load a clamped row and are then replaced by the ghost
+    // values (0, inactive), so the loads need no branch.
     const int kbeg = r0 - 3, kend = r1 + 3;
-    T nh = 0, nu = 0, nv = 0, nz = 0;
-    uint8_t ncl = 0;
-    {
-        const int k = kbeg;
-        if (k >= 0 && k < R && colin) {
-            const long off = (long)k * P + j;
-            nh = a.h_in[off]; nu = a.um_in[off]; nv = a.vn_in[off]; nz = a.z[off];
-            if (s.ncls_on) ncl = a.cls[off];
-        } else {
-            nz = __builtin_nan("");
-        }
-    }
+    const int jc = colin ? j : (j < 0 ? 0 : C - 1);
+    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
+        const bool in = colin && k >= 0 && k < R;
+        const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
+        const int off = kc * P + jc;
+        const T lh = a.h_in[off], lu = a.um_in[off], lv = a.vn_in[off];
+        const double lz = a.z[off];
+        hh = in ? lh : T(0);
+        uu = in ? lu : T(0);
+        vv = in ? lv : T(0);
+        zz = in ? lz : __builtin_nan("");
+        cc = (s.ncls_on && in) ? a.cls[off] : (uint8_t)0;
+    };
+    T nh, nu, nv;
+    double nz;
+    uint8_t ncl;
+    load_row(kbeg, nh, nu, nv, nz, ncl);
 
     for (int k = kbeg; k < kend; ++k) {
-        const T h0 = nh, u0 = nu, v0 = nv, z0 = nz;
+        const T h0 = nh, u0 = nu, v0 = nv;
+        const double z0 = nz;
         const uint8_t cl0 = ncl;
-        if (k + 1 < kend) {
-            const int kn = k + 1;
-            if (kn >= 0 && kn < R && colin) {
-                const long off = (long)kn * P + j;
-                nh = a.h_in[off]; nu = a.um_in[off]; nv = a.vn_in[off]; nz = a.z[off];
-                if (s.ncls_on) ncl = a.cls[off];
-            } else {
-                nh = 0; nu = 0; nv = 0; nz = __builtin_nan(""); ncl = 0;
-            }
-        }
+        load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
         const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
         const T h0w = from_west(h0);
-        const T z0w = from_west(z0);
+        const double z0w = from_west(z0);
         const bool a0w = west_bit(a0, lane);
         const T u0e = from_east(u0), u0w = from_west(u0);
         const T v0e = from_east(v0), v0w = from_west(v0);
         const T nc0w = from_west(nc0);
         const T hn0 = (j == 0) ? h0 : T(0.50) * (h0 + h0w);
         const T hm0 = (k == 0) ? h0 : ((k == R) ? h1 : T(0.50) * (h0 + h1));
-        const T uu1_0 = (hm0 >= th) ? u0 / hm0 : T(0.00);
-        const T vv1_0 = (hn0 >= th) ? v0 / hn0 : T(0.0);
+        const T uu1_0 = (hm0 >= th) ? M::div_(u0, hm0) : T(0.00);
+        const T vv1_0 = (hn0 >= th) ? M::div_(v0, hn0) : T(0.0);
         const T vv1_0e = from_east(vv1_0), vv1_0w = from_west(vv1_0);
         const T uu1_0w = from_west(uu1_0);
         // uua_vva (489-522): vva of row k, uua of row k-1
@@ -170,12 +177,12 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         if (ADAPT && colout && k >= r0 && k < r1) {
             // CFL candidates of the incoming state on the faces of row k
             if (k >= 1 && a0 && a1 && hm0 > th) {
-                const double w = s.dx / ((double)M::abs_(uu1_0) + sqrt(s.g * (double)hm0));
-                cfl = (w < cfl) ? w : cfl;
+                const double w = (double)M::abs_(uu1_0) + sqrt(s.g * (double)hm0);
+                smax_x = (w > smax_x) ? w : smax_x;
             }
             if (j >= 1 && a0 && a0w && hn0 > th) {
-                const double w = s.dy / ((double)M::abs_(vv1_0) + sqrt(s.g * (double)hn0));
-                cfl = (w < cfl) ? w : cfl;
+                const double w = (double)M::abs_(vv1_0) + sqrt(s.g * (double)hn0);
+                smax_y = (w > smax_y) ? w : smax_y;
             }
         }
 
@@ -184,73 +191,77 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         T umn = 0, vnn = 0;
         // rows before r0 - 1 feed no output row: the warm-up only builds the window
         const bool flux_um = k >= r0, flux_vn = k >= r0 && k <= r1 + 1;
-        // x direction: um at the north face of (f, j); w = row f-1, e = row f
-        if (flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th))) {
-            const T hhe = h1 + z1, hhw = h2 + z2, hhep = h1 - th, hhwp = h2 - th;
-            if (hhe < z2) {
-                umn = (h2 > th) ? T(0.5440) * h2 * M::sqrt_(gg * h2) : T(0);
-            } else if (hhw < z1) {
-                umn = (h1 > th) ? T(-0.544) * h1 * M::sqrt_(gg * h1) : T(0);
-            } else if (hhep * hhwp < 0) {
-                if ((h1 > 0) || (h2 > 0)) {
-                    const T hhan = hhep - hhwp;
-                    const T sgnm = hhan / M::abs_(hhan);
-                    const T hh3 = M::max_((h1 + z1), (h2 + z2)) - M::max_(z1, z2);
-                    umn = -sgnm * T(0.350) * hh3 * M::sqrt_(T(2.00) * gg * hh3);
-                } else {
-                    umn = 0;
-                }
-            } else {
-                const T u13 = gg * hm1 * dt2dx * (h1 + z1 - h2 - z2);
+        // x direction: um at the north face of (f, j); w = row f-1, e = row f.
+        // Dry-bed branches B1-B3 all read (coef * hv) * sqrt(c2 * hv); B4 (the
+        // momentum update) runs only when a lane of the wave needs it.
+        const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th));
+        if (__any(act_x)) {
+            const double hhe = h1 + z1, hhw = h2 + z2;  // water surfaces, FP64
+            const T hhep = h1 - th, hhwp = h2 - th;
+            const bool b1 = hhe < z2, b2 = !b1 && (hhw < z1);
+            const bool b3 = !b1 && !b2 && (hhep * hhwp < 0);
+            const bool b4 = act_x && !b1 && !b2 && !b3;
+            T coef = T(0.5440), hv = h2, c2 = gg;
+            bool wet = h2 > th;
+            if (b2) { coef = T(-0.544); hv = h1; wet = h1 > th; }
+            if (b3) {
+                const T sgnm = M::sign_(hhep - hhwp);  // hhan / |hhan|
+                coef = -sgnm * T(0.350);
+                hv = (T)(__builtin_fmax(hhe, hhw) - __builtin_fmax(z1, z2));
+                c2 = T(2.00) * gg;
+                wet = (h1 > 0) || (h2 > 0);
+            }
+            T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
+            if (__any(b4)) {
+                const T u13 = gg * hm1 * dt2dx * (T)(hhe - h2 - z2);
                 const T ur = T(0.50) * (uu1_0 + uu1_1);
                 const T ul = T(0.50) * (uu1_1 + uu1_2);
                 const T umr = ur * (u0 + u1) * T(0.50) + M::abs_(ur) * (u1 - u0) * T(0.50);
                 const T uml = ul * (u1 + u2) * T(0.50) + M::abs_(ul) * (u2 - u1) * T(0.50);
                 const T u11 = dt2dx * (umr - uml);
-                T u12 = 0;
-                if (j != 0) {
-                    const T vu = T(0.50) * (vv1_1e + vv1_2e);
-                    const T vd = T(0.50) * (vv1_1 + vv1_2);
-                    const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
-                    const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
-                    u12 = dt2dy * (umu - umd);
-                }
+                const T vu = T(0.50) * (vv1_1e + vv1_2e);
+                const T vd = T(0.50) * (vv1_1 + vv1_2);
+                const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
+                const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
+                const T u12 = (j != 0) ? dt2dy * (umu - umd) : T(0);
                 const T nf = s.ncls_on ? T(0.5) * (nc2 + nc1) : rn;
                 T ram = 0.00;
-                if (!(hm1 <= th)) {
+                if (b4 && !(hm1 <= th)) {
                     const T sqx = M::sqrt_(uu1_1 * uu1_1 + vva1 * vva1);
-                    ram = gg * nf * nf * sqx / pow1333(hm1);
+                    ram = M::div_(gg * nf * nf * sqx, pow1333(hm1));
                 }
-                umn = ((T(1.00) - hdt2 * ram * T(0.50)) * u1 + (-u11 - u12 - u13)) /
-                      (T(1.00) + hdt2 * ram * T(0.50));
+                const T up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * u1 + (-u11 - u12 - u13),
+                                     T(1.00) + dt2 * ram * T(0.50));
+                if (b4) val = up;
             }
+            umn = act_x ? val : T(0);
         }
         // y direction: vn at the west face of (f, j); s = column j-1, n = column j
-        if (flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th))) {
-            const T hhn = h1 + z1, hhs = h1w + z1w, hhnp = h1 - th, hhsp = h1w - th;
-            if (hhn < z1w) {
-                vnn = (h1w > th) ? T(0.5440) * h1w * M::sqrt_(gg * h1w) : T(0);
-            } else if (hhs < z1) {
-                vnn = (h1 > th) ? T(-0.544) * h1 * M::sqrt_(gg * h1) : T(0);
-            } else if (hhnp * hhsp < 0) {
-                if ((h1 > 0) || (h1w > 0)) {
-                    const T hhan = hhnp - hhsp;
-                    const T sgnm = hhan / M::abs_(hhan);
-                    const T hh3 = M::max_((h1 + z1), (h1w + z1w)) - M::max_(z1, z1w);
-                    vnn = -sgnm * T(0.350) * hh3 * M::sqrt_(T(2.00) * gg * hh3);
-                } else {
-                    vnn = 0;
-                }
-            } else {
-                const T v13 = gg * hn1 * dt2dy * (h1 + z1 - h1w - z1w);
-                T v11 = 0;
-                if (f != 0 && f != R) {
-                    const T av = T(0.50) * (uu1_0 + uu1_0w);
-                    const T bv = T(0.50) * (uu1_1 + uu1_1w);
-                    const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
-                    const T vnl = bv * (v1 + v2) * T(0.50) + M::abs_(bv) * (v2 - v1) * T(0.50);
-                    v11 = dt2dx * (vnr - vnl);
-                }
+        const bool act_y = flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th));
+        if (__any(act_y)) {
+            const double hhn = h1 + z1, hhs = h1w + z1w;
+            const T hhnp = h1 - th, hhsp = h1w - th;
+            const bool b1 = hhn < z1w, b2 = !b1 && (hhs < z1);
+            const bool b3 = !b1 && !b2 && (hhnp * hhsp < 0);
+            const bool b4 = act_y && !b1 && !b2 && !b3;
+            T coef = T(0.5440), hv = h1w, c2 = gg;
+            bool wet = h1w > th;
+            if (b2) { coef = T(-0.544); hv = h1; wet = h1 > th; }
+            if (b3) {
+                const T sgnm = M::sign_(hhnp - hhsp);
+                coef = -sgnm * T(0.350);
+                hv = (T)(__builtin_fmax(hhn, hhs) - __builtin_fmax(z1, z1w));
+                c2 = T(2.00) * gg;
+                wet = (h1 > 0) || (h1w > 0);
+            }
+            T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
+            if (__any(b4)) {
+                const T v13 = gg * hn1 * dt2dy * (T)(hhn - h1w - z1w);
+                const T av = T(0.50) * (uu1_0 + uu1_0w);
+                const T bv = T(0.50) * (uu1_1 + uu1_1w);
+                const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
+                const T vnl = bv * (v1 + v2) * T(0.50) + M::abs_(bv) * (v2 - v1) * T(0.50);
+                const T v11 = (f != 0 && f != R) ? dt2dx * (vnr - vnl) : T(0);
                 const T pv = T(0.50) * (vv1_1e + vv1_1);
                 const T qv = T(0.50) * (vv1_1 + vv1_1w);
                 const T vnu = pv * (v1e + v1) * T(0.50) + M::abs_(pv) * (v1 - v1e) * T(0.50);
@@ -258,13 +269,15 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
                 const T v12 = dt2dy * (vnu - vnd);
                 const T nf = s.ncls_on ? T(0.5) * (nc1w + nc1) : rn;
                 T ram = 0.00;
-                if (!(hn1 <= th)) {
+                if (b4 && !(hn1 <= th)) {
                     const T sqy = M::sqrt_(uua1 * uua1 + vv1_1 * vv1_1);
-                    ram = gg * nf * nf * sqy / pow1333(hn1);
+                    ram = M::div_(gg * nf * nf * sqy, pow1333(hn1));
                 }
-                vnn = ((T(1.00) - hdt2 * ram * T(0.50)) * v1 + (-v11 - v12 - v13)) /
-                      (T(1.00) + hdt2 * ram * T(0.50));
+                const T up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * v1 + (-v11 - v12 - v13),
+                                     T(1.00) + dt2 * ram * T(0.50));
+                if (b4) val = up;
             }
+            vnn = act_y ? val : T(0);
         }
 
         // ---- first continuity of row g = k-2 at time t (602-664) ----
@@ -272,9 +285,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         bool dryg = false;
         T umpg = 0, vnpg = 0;
         if (k >= r0 + 1) {
-        const T vnnPe = from_east(vnnP);
-        T h1v;
-        {
+            const T vnnPe = from_east(vnnP);
             const bool wet = h2 > ewl;
             const T Pc = wet ? P_lake : P_land;
             const T E = wet ? (T)ev1 : T(0.000);
@@ -295,26 +306,26 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
                 else vne = qo;
                 qo_eff = qo;
             }
-            const T hr = h2 - dt2 * ((ums - umnn) / dx + (vne - vnw) / dy - ql - rr * Pc + E);
-            h1v = M::max_(hr, T(0.00));
+            const T hr = h2 - dt2 * (M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) -
+                                     ql - rr * Pc + E);
+            T h1v = M::max_(hr, T(0.00));
             if (!a2) h1v = 0.00;
             dryg = a2 && (h1v < th);
-        }
-        // ---- treat_error on the faces of row g (695-725) ----
-        umpg = umnP;
-        if (dryP && umpg > 0) umpg = 0;
-        if (dryg && umpg < 0) umpg = 0;
-        const bool drygw = west_bit(dryg, lane);
-        vnpg = vnnP;
-        if (drygw && vnpg > 0) vnpg = 0;
-        if (dryg && vnpg < 0) vnpg = 0;
-        if (is_out_col && g == s.out_r && dryg) qo_eff = 0;
+            // ---- treat_error on the faces of row g (695-725) ----
+            umpg = umnP;
+            if (dryP && umpg > 0) umpg = 0;
+            if (dryg && umpg < 0) umpg = 0;
+            const bool drygw = west_bit(dryg, lane);
+            vnpg = vnnP;
+            if (drygw && vnpg > 0) vnpg = 0;
+            if (dryg && vnpg < 0) vnpg = 0;
+            if (is_out_col && g == s.out_r && dryg) qo_eff = 0;
         }
 
         // ---- second continuity of row q = k-3 at time t + dt; store ----
         const int q = k - 3;
         if (k >= r0 + 3) {
-        const T vnpPe = from_east(vnpP);
+            const T vnpPe = from_east(vnpP);
             const bool wet = h3 > ewl;
             const T Pc = wet ? P_lake : P_land;
             const T E = wet ? (T)ev2 : T(0.000);
@@ -334,7 +345,8 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
                 else if (s.out_edge == 3) vnw = -qo_eff;
                 else vne = qo_eff;
             }
-            const T hr = h3 - dt2 * ((ums - umnn) / dx + (vne - vnw) / dy - ql - rr * Pc + E);
+            const T hr = h3 - dt2 * (M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) -
+                                     ql - rr * Pc + E);
             T h2v = M::max_(hr, T(0.00));
             if (!a3) h2v = 0.00;
             if (colout && q >= r0 && q < r1) {
@@ -370,7 +382,14 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     acc_src = wave_sum(acc_src);
     acc_clip = wave_sum(acc_clip);
     acc_q = wave_sum(acc_q);
-    if (ADAPT) cfl = wave_min(cfl);
+    double cfl = INFINITY;
+    if (ADAPT) {
+        smax_x = wave_max(smax_x);
+        smax_y = wave_max(smax_y);
+        const double wx = smax_x > 0.0 ? s.dx / smax_x : INFINITY;
+        const double wy = smax_y > 0.0 ? s.dy / smax_y : INFINITY;
+        cfl = (wx < wy) ? wx : wy;
+    }
     if (lane == 0) {
         Partial p;
         p.vol = acc_vol; p.src = acc_src; p.clip = acc_clip; p.qout = acc_q;
@@ -380,11 +399,11 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     }
 }
 
-// initial CFL minimum of the state (adaptive mode), one wave per tile rows
+// initial CFL minimum of the state (adaptive mode)
 template <typename T>
-__global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T *vn, const T *z,
-                                                int R, int C, int P, double dx, double dy,
-                                                double g, double th, Clock *clk)
+__global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T *vn,
+                                                const double *z, int R, int C, int P, double dx,
+                                                double dy, double g, double th, Clock *clk)
 {
     using M = num<T>;
     const long n = (long)R * C;
@@ -398,7 +417,7 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
             const bool an = (z[o - P] == z[o - P]);
             const T hm = T(0.50) * (h[o] + h[o - P]);
             if (ac && an && hm > (T)th) {
-                const T uu1 = (hm >= (T)th) ? um[o] / hm : T(0.00);
+                const T uu1 = (hm >= (T)th) ? M::div_(um[o], hm) : T(0.00);
                 const double w = dx / ((double)M::abs_(uu1) + sqrt(g * (double)hm));
                 cfl = (w < cfl) ? w : cfl;
             }
@@ -407,7 +426,7 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
             const bool aw = (z[o - 1] == z[o - 1]);
             const T hn = T(0.50) * (h[o] + h[o - 1]);
             if (ac && aw && hn > (T)th) {
-                const T vv1 = (hn >= (T)th) ? vn[o] / hn : T(0.0);
+                const T vv1 = (hn >= (T)th) ? M::div_(vn[o], hn) : T(0.0);
                 const double w = dy / ((double)M::abs_(vv1) + sqrt(g * (double)hn));
                 cfl = (w < cfl) ? w : cfl;
             }
@@ -421,8 +440,8 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
 // one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva)
 template <typename T>
 __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
-                                                    const T *z, int R, int C, int P, double th,
-                                                    int field, double *out)
+                                                    const double *z, int R, int C, int P,
+                                                    double th, int field, double *out)
 {
     PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
     const long n = (long)R * C;
@@ -430,7 +449,7 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
          idx += (long)gridDim.x * blockDim.x) {
         const int r = (int)(idx / C), j = (int)(idx % C);
         const long o = (long)r * P + j;
-        T v = 0;
+        double v = 0;
         switch (field) {
         case SW2D_FIELD_H: v = h[o]; break;
         case SW2D_FIELD_UM: v = um[o]; break;
@@ -446,23 +465,18 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
         case SW2D_FIELD_Z: v = z[o]; break;
         default: break;
         }
-        out[idx] = (double)v;
+        out[idx] = v;
     }
 }
 
-__global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *out)
-{
-    ledger_peek(s, out, threadIdx.x);
-}
-
 template <typename T>
-static hipError_t launch_fused_t(sw2d_handle *h, const StepCommon &sc)
+static hipError_t launch_fused_impl(sw2d_handle *h, const StepCommon &sc)
 {
     FusedArgs<T> a;
     const int in = h->cur, outb = 1 - h->cur;
     a.h_in = (const T *)h->h[in]; a.um_in = (const T *)h->um[in]; a.vn_in = (const T *)h->vn[in];
     a.h_out = (T *)h->h[outb]; a.um_out = (T *)h->um[outb]; a.vn_out = (T *)h->vn[outb];
-    a.z = (const T *)h->z;
+    a.z = (const double *)h->z;
     a.cls = h->cls;
     a.P = h->P; a.W = h->W; a.H = h->H;
     a.nstrips = h->nstrips; a.ntiles = h->ntiles;
@@ -477,52 +491,29 @@ static hipError_t launch_fused_t(sw2d_handle *h, const StepCommon &sc)
     return hipGetLastError();
 }
 
-hipError_t launch_fused(sw2d_handle *h, const StepCommon &sc)
-{
-    if (h->esize == 8) return launch_fused_t<double>(h, sc);
-    return launch_fused_t<float>(h, sc);
-}
-
-hipError_t launch_init(sw2d_handle *h)
+template <typename T>
+static hipError_t launch_init_impl(sw2d_handle *h)
 {
     if (!h->prm.adaptive) return hipSuccess;
     const long n = (long)h->R * h->C;
     int blocks = (int)((n + 255) / 256);
     if (blocks > 4096) blocks = 4096;
-    if (h->esize == 8)
-        hipLaunchKernelGGL(init_cfl<double>, dim3(blocks), dim3(256), 0, h->stream,
-                           (const double *)h->h[h->cur], (const double *)h->um[h->cur],
-                           (const double *)h->vn[h->cur], (const double *)h->z, h->R, h->C, h->P,
-                           h->grid.cellsize, h->grid.cellsize, h->prm.g, h->prm.th, h->clk);
-    else
-        hipLaunchKernelGGL(init_cfl<float>, dim3(blocks), dim3(256), 0, h->stream,
-                           (const float *)h->h[h->cur], (const float *)h->um[h->cur],
-                           (const float *)h->vn[h->cur], (const float *)h->z, h->R, h->C, h->P,
-                           h->grid.cellsize, h->grid.cellsize, h->prm.g, h->prm.th, h->clk);
+    hipLaunchKernelGGL(init_cfl<T>, dim3(blocks), dim3(256), 0, h->stream,
+                       (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
+                       (const double *)h->z, h->R, h->C, h->P, h->grid.cellsize, h->grid.cellsize,
+                       h->prm.g, h->prm.th, h->clk);
     return hipGetLastError();
 }
 
-hipError_t launch_derive(sw2d_handle *h, int field, double *out_dev)
+template <typename T>
+static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev)
 {
     const long n = (long)h->R * h->C;
     int blocks = (int)((n + 255) / 256);
     if (blocks > 8192) blocks = 8192;
-    if (h->esize == 8)
-        hipLaunchKernelGGL(derive_field<double>, dim3(blocks), dim3(256), 0, h->stream,
-                           (const double *)h->h[h->cur], (const double *)h->um[h->cur],
-                           (const double *)h->vn[h->cur], (const double *)h->z, h->R, h->C, h->P,
-                           h->prm.th, field, out_dev);
-    else
-        hipLaunchKernelGGL(derive_field<float>, dim3(blocks), dim3(256), 0, h->stream,
-                           (const float *)h->h[h->cur], (const float *)h->um[h->cur],
-                           (const float *)h->vn[h->cur], (const float *)h->z, h->R, h->C, h->P,
-                           h->prm.th, field, out_dev);
-    return hipGetLastError();
-}
-
-hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc)
-{
-    hipLaunchKernelGGL(peek_kernel, dim3(1), dim3(64), 0, h->stream, sc, h->peek);
+    hipLaunchKernelGGL(derive_field<T>, dim3(blocks), dim3(256), 0, h->stream,
+                       (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
+                       (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev);
     return hipGetLastError();
 }
 

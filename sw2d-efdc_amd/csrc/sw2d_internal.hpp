@@ -81,7 +81,7 @@ template <typename T>
 struct FusedArgs {
     const T *h_in; const T *um_in; const T *vn_in;
     T *h_out; T *um_out; T *vn_out;
-    const T *z;              // NaN marks an inactive cell
+    const double *z;         // bed elevation (FP64 in both forms); NaN marks an inactive cell
     const uint8_t *cls;      // Manning class per cell (ncls_on)
     int P;                   // row pitch in elements
     int W;                   // output columns per tile (64 - 6)
@@ -147,11 +147,16 @@ struct sw2d_handle {
 };
 
 namespace sw2d {
-// launchers (sw2d_fused.hip, sw2d_plain.hip)
-hipError_t launch_fused(sw2d_handle *h, const StepCommon &sc);
-hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc);
-hipError_t launch_init(sw2d_handle *h);
-hipError_t launch_derive(sw2d_handle *h, int field, double *out_dev);
+// launchers (sw2d_fused64.hip, sw2d_fused32.hip, sw2d_plain.hip)
+hipError_t launch_fused_f64(sw2d_handle *h, const StepCommon &sc);
+hipError_t launch_fused_f32(sw2d_handle *h, const StepCommon &sc);
+hipError_t launch_init_f64(sw2d_handle *h);
+hipError_t launch_init_f32(sw2d_handle *h);
+hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev);
+hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev);
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc);
+int fused_waves_per_cu_f64(int adaptive);  // resident one-wave blocks per CU
+int fused_waves_per_cu_f32(int adaptive);
+hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc);
 hipError_t launch_plain_init(sw2d_handle *h);
 }  // namespace sw2d

@@ -1,0 +1,85 @@
+// sw2d_io.hpp -- the reference's case-file contract (db/<case>/...), read and
+// written by the sw2d-efdc driver.  Input parsing keeps the reference's
+// fscanf semantics (whitespace in a format matches any run of whitespace,
+// other characters must match literally, a failed conversion leaves the
+// target unchanged); outputs keep its printf formats byte for byte.
+#pragma once
+#include <stdio.h>
+
+#include <string>
+#include <vector>
+
+namespace sw2dio {
+
+// 1-SETUP and 2-SETUP blocks of setup.dat (SW2D-EFDC.cu:3763-3969)
+struct Setup {
+    double tday0 = 0, thour0 = 0, tmin0 = 0, tsec0 = 0;
+    double tday = 0, thour = 0, tmin = 0, tsec = 0;
+    double dt = 0, dpout = 0, dkout = 0, dtoq = 0;
+    int evaporation_on = 0;
+    double EV_WL_min = 0, INT = 0, INF = 0, LWL = 0, manning = 0;
+    int out_vx = 0, out_vy = 0, out_elev = 0, out_depth = 0, out_outlet = 0;
+    int efdc = 0;
+    // EFDC block: only the values the SW2D side uses are kept by name
+    double efdc_basetime = 0, TBEGIN = 0;
+    int NTSPTC = 0, nlayers = 1;
+    std::vector<double> efdc_block;  // every value of the block, in file order
+};
+
+struct Grid {
+    int cols = 0, rows = 0;
+    double xll = 0, yll = 0, cellsize = 0, nodata = 0;
+    std::vector<double> v;  // rows*cols values, north row first
+};
+
+struct Case {
+    std::string dir;  // db/<case>
+    Setup setup;
+    Grid dem;
+    std::vector<double> src_x, src_y;  // coord_source_sink.dat (metres)
+    std::vector<double> Q;             // Q_source_sink.dat, nq rows of nst values
+    int nq = 0;
+    double dtrain = 0;
+    std::vector<double> rain;          // rain.dat, mm per dtrain (cont_rain - 1 values)
+    std::vector<double> T, Rg;         // temperatures / solar radiation
+    double albedo = 0;
+    std::vector<double> h0;            // initial_condition.dat
+    std::vector<double> out_x, out_y;  // coord_out.dat (metres)
+    std::vector<int> mask;             // mask.dat (efdc == 1)
+    Grid mask_hdr;                     // mask.dat header (efdc == 1)
+};
+
+// count of '\n' in a file (the reference sizes series this way, 4041-4048)
+int count_newlines(const std::string &path);
+
+// db/dir.txt: case names
+std::string read_dir_list(const std::string &root, std::vector<std::string> &names);
+
+// read a whole case; returns "" or an error naming the missing file
+std::string read_case(const std::string &root, const std::string &name, Case &c);
+
+// VTK STRUCTURED_POINTS writer (SW2D-EFDC.cu:4958-5072): fields are rows*cols
+// in memory order, written south row first with "%f"; a NULL field is skipped
+struct VtkFields {
+    const double *depth = nullptr, *vv = nullptr, *uu = nullptr, *z = nullptr;
+};
+bool write_vtk(const std::string &path, int cols, int rows, double dx, double dy, double x0,
+               double y0, const VtkFields &f);
+
+// one WaterLevel.txt output (4986-5013): probes are matched in memory order
+// and print h at their cell with " %lf", the last one with a newline
+void write_water_level(FILE *fp, int rows, int cols, double xcoor_cells, double ycoor_cells,
+                       const std::vector<double> &outx_cells,
+                       const std::vector<double> &outy_cells, const double *h);
+
+// QSER.INP (write_efdc_QSER, 2499-2610) for outputs 0..tq; per edge list
+// (0 N, 1 S, 2 E, 3 W): time[t] and val[t][record]
+struct QserSeries {
+    std::vector<double> time[4];
+    std::vector<std::vector<double>> val[4];
+    int nrec[4] = {0, 0, 0, 0};
+};
+bool write_qser(const std::string &path, int type, int nlayers, const QserSeries &q,
+                double efdc_basetime, int tq);
+
+}  // namespace sw2dio

@@ -103,18 +103,29 @@ __device__ __forceinline__ float exp_f(float t)
     return __builtin_amdgcn_exp2f(t * 1.44269504f);
 }
 
+// Arithmetic of the state type.  FP64: the IEEE operations, exactly as the
+// reference writes them.  FP32 (precision form): hardware reciprocal and
+// square root (about 1 ulp), division by a constant as a multiplication.
 template <typename T> struct num;
 template <> struct num<double> {
     __device__ static __forceinline__ double sqrt_(double x) { return __builtin_sqrt(x); }
     __device__ static __forceinline__ double abs_(double x) { return __builtin_fabs(x); }
     __device__ static __forceinline__ double max_(double a, double b) { return __builtin_fmax(a, b); }
     __device__ static __forceinline__ double exp_(double x) { return exp_d(x); }
+    __device__ static __forceinline__ double div_(double a, double b) { return a / b; }
+    // x / |x| for nonzero finite x (exactly +-1)
+    __device__ static __forceinline__ double sign_(double x) { return __builtin_copysign(1.0, x); }
+    // a / b where rb = 1 / b is precomputed (FP64 still divides)
+    __device__ static __forceinline__ double divc_(double a, double b, double rb) { return a / b; }
 };
 template <> struct num<float> {
-    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_sqrtf(x); }
+    __device__ static __forceinline__ float sqrt_(float x) { return __builtin_amdgcn_sqrtf(x); }
     __device__ static __forceinline__ float abs_(float x) { return __builtin_fabsf(x); }
     __device__ static __forceinline__ float max_(float a, float b) { return __builtin_fmaxf(a, b); }
     __device__ static __forceinline__ float exp_(float x) { return exp_f(x); }
+    __device__ static __forceinline__ float div_(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+    __device__ static __forceinline__ float sign_(float x) { return __builtin_copysignf(1.0f, x); }
+    __device__ static __forceinline__ float divc_(float a, float b, float rb) { return a * rb; }
 };
 
 // stream_flow's interpolation (SW2D-EFDC.cu:532-537, 546-551); reads past the

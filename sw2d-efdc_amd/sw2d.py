@@ -84,6 +84,7 @@ def lib():
         "sw2d_set_outlet": (ct.c_int, [_H, ct.c_int, ct.c_int]),
         "sw2d_set_hydrograph": (ct.c_int, [_H, ct.c_long]),
         "sw2d_set_lake_bc": (ct.c_int, [_H, _IP]),
+        "sw2d_autotune": (ct.c_int, [_H, ct.c_int, _IP]),
         "sw2d_step": (ct.c_int, [_H, ct.c_int]),
         "sw2d_run_until": (ct.c_int, [_H, _D, ct.c_long, ct.POINTER(ct.c_long)]),
         "sw2d_sync": (ct.c_int, [_H]),
@@ -252,6 +253,12 @@ class Case:
         n = ct.c_int(0)
         self._chk(self._L.sw2d_lake_inflow(self._h, None, 0, ct.byref(n)), "lake_inflow")
         self.nbc = n.value
+
+    def autotune(self, iters=20):
+        """Measured choice of the fused kernel's tile height (before stepping)."""
+        best = ct.c_int(0)
+        self._chk(self._L.sw2d_autotune(self._h, iters, ct.byref(best)), "autotune")
+        return best.value
 
     def step(self, n=1):
         self._chk(self._L.sw2d_step(self._h, n), "step")

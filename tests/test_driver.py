@@ -1,0 +1,179 @@
+"""The sw2d-efdc drop-in driver: the reference's db/ file contract.
+
+CPU tests parse generated cases (--parse-only).  The GPU test runs a whole
+coupled case and compares every output file with the same files built from
+the CPU FP64 oracle, formatted with the reference's printf formats
+(SW2D-EFDC.cu:4958-5072 for VTK and WaterLevel.txt, 2499-2610 for QSER.INP).
+"""
+import json
+import os
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+DRIVER = ROOT / "sw2d-efdc_amd" / "sw2d-efdc"
+
+
+def small_case(cases, R=24, C=37, t_end=480.0):
+    spec = cases.config0(rows=R, cols=C, seed=11)
+    spec.t_end = t_end
+    spec.rain = np.array([60.0, 30.0, 30.0, 0.0, 0.0])
+    spec.dtrain = 120.0
+    mask = np.zeros((R, C), np.int32)
+    mask[R // 2:R // 2 + 4, C // 3:C // 3 + 6] = 1
+    spec.lake_mask = mask
+    return spec
+
+
+CASE_KW = dict(dkout=120.0, dpout=60.0, probes=[(55.0, 105.0), (125.0, 35.0), (305.0, 205.0)],
+               sources=([105.0], [155.0], [[0.5], [1.0], [0.2], [0.0]], 150.0), efdc=True,
+               nlayers=2, basetime=3600.0,
+               evaporation=(0.2, [20.0, 21.0, 22.0, 23.0, 24.0], [300.0, 400, 500, 600, 700]),
+               INT=0.1, INF=0.05, LWL=0.2, EV_WL_min=0.01)
+
+
+def run_driver(root, *args, check=True):
+    r = subprocess.run([str(DRIVER), "--root", str(root), *args], capture_output=True, text=True,
+                       timeout=600)
+    if check and r.returncode != 0:
+        raise AssertionError(f"sw2d-efdc failed ({r.returncode}): {r.stdout[-2000:]}{r.stderr[-2000:]}")
+    return r
+
+
+def test_parse_only_roundtrip(cases, tmp_path):
+    spec = small_case(cases)
+    cases.write_db(spec, tmp_path, "C1", **CASE_KW)
+    r = run_driver(tmp_path, "--parse-only")
+    j = json.loads(r.stdout.strip().splitlines()[-1])
+    assert (j["rows"], j["cols"], j["cellsize"], j["nodata"]) == (24, 37, 10.0, -9999.0)
+    assert (j["dt"], j["dkout"], j["dpout"], j["dtoq"]) == (0.5, 120.0, 60.0, 150.0)
+    assert (j["manning"], j["INT"], j["INF"], j["LWL"], j["EV_WL_min"]) == (0.03, 0.1, 0.05, 0.2, 0.01)
+    assert (j["efdc"], j["efdc_basetime"], j["nlayers"], j["evaporation"]) == (1, 3600.0, 2, 1)
+    assert (j["nst"], j["nq"], j["nrain"], j["dtrain"], j["n_out"]) == (1, 4, 5, 120.0, 3)
+    assert j["rain_sum"] == 120.0 and j["timmax"] == 480.0 and j["time0"] == 0.0
+    assert j["z_sum"] == pytest.approx(spec.z.sum(), rel=1e-14)
+    assert j["mask_lake"] == int(spec.lake_mask.sum())
+
+
+def test_setup_crlf_and_whitespace(cases, tmp_path):
+    """setup.dat is matched like fscanf: any whitespace run (CRLF included)
+    between the comment lines and values is accepted."""
+    spec = small_case(cases)
+    base = cases.write_db(spec, tmp_path, "C2", **CASE_KW)
+    p = os.path.join(base, "setup.dat")
+    txt = open(p).read().replace("\n", "   \r\n")
+    open(p, "w", newline="").write(txt)
+    j = json.loads(run_driver(tmp_path, "--parse-only").stdout.strip().splitlines()[-1])
+    assert (j["dt"], j["manning"], j["efdc_basetime"], j["nlayers"]) == (0.5, 0.03, 3600.0, 2)
+
+
+def test_missing_input_is_reported(cases, tmp_path):
+    spec = small_case(cases)
+    base = cases.write_db(spec, tmp_path, "C3", **CASE_KW)
+    os.remove(os.path.join(base, "SW2D", "input", "dem.dat"))
+    r = run_driver(tmp_path, "--parse-only", check=False)
+    assert r.returncode != 0 and "unknown file - dem.dat" in r.stdout
+
+
+# ---- expected files from the oracle (formats of the reference's printf calls) ----
+
+def vtk_text(R, C, dx, depth, vv, uu, z):
+    out = ["# vtk DataFile Version 2.0", "Brazil", "ASCII", "DATASET STRUCTURED_POINTS",
+           f"DIMENSIONS {C} {R} 1", "ASPECT_RATIO %f %f %f" % (dx, dx, 1.0),
+           "ORIGIN %f %f %f" % (0.0, 0.0, 0.0), f"POINT_DATA {C * R}"]
+    for name, a in (("depth", depth), ("x_velocity", vv), ("y_velocity", -uu),
+                    ("Elevations", z)):
+        out += [f"SCALARS {name} float 1", "LOOKUP_TABLE default"]
+        out += ["%f" % v for v in a[::-1].ravel()]
+    return "\n".join(out) + "\n"
+
+
+def c_round(v):
+    """C round(): halves away from zero (Python's round() goes to even)."""
+    return int(np.sign(v) * np.floor(abs(v) + 0.5))
+
+
+def water_level_line(R, C, dx, probes, h):
+    idx = []
+    for x, y in probes:
+        outx = c_round(abs(0.0 - x / dx))
+        outy = R - c_round(abs(0.0 - y / dx))
+        idx.append(outy * C - (C - (outx + 1)))
+    vals = [" %f" % h.ravel()[i] for i in sorted(idx)]
+    return "".join(vals) + "\n"
+
+
+def qser_text(hist, nlayers, basetime):
+    hdr = ["C **, qser.inp Time Series FILE", "C **",
+           "C **  InType MQSER(NS) TCQSER(NS) TAQSER(NS) RMULADJ(NS) ADDADJ(NS) ICHGQS", "C **",
+           "C **  IF InType.EQ.1 THEN READ DEPTH WEIGHTS AND SINGLE VALUE OF QSER",
+           "C **                      ELSE READ A VALUE OF QSER FOR EACH LAYER", "C **",
+           "C **  InType = 1 Structure", "C **  WKQ(K), K = 1, KC",
+           "C **  TQSER(M, NS)    QSER(M, 1, NS)          !(MQSER(NS) PAIRS FOR NS = 1, NQSER SERIES)",
+           "C **", "C **  InType = 0 Structure",
+           "C **  TQSER(M, NS)    (QSER(M, K, NS), K = 1, KC) !(MQSER(NS) PAIRS)", "C **"]
+    lines = list(hdr)
+    tq = len(hist) - 1
+    edg = 0
+    for edge, label in (("north", "North"), ("south", "South"), ("east", "East"), ("west", "West")):
+        n = len(hist[0][edge]["q"])
+        for b in range(n):
+            edg += 1
+            lines.append("      0 %d %f 0 1 0 0  ' *** %s_edge_%d" % (tq + 1, basetime, label, edg))
+            for t in range(tq + 1):
+                v = hist[t][edge]["q"][b] / nlayers
+                lines.append("      %f " % hist[t]["time"] + " ".join(["%f" % v] * nlayers))
+    return "\n".join(lines) + "\n"
+
+
+@pytest.mark.gpu
+def test_driver_outputs_match_oracle(cases, po, tmp_path):
+    spec = small_case(cases)
+    base = cases.write_db(spec, tmp_path, "C4", **CASE_KW)
+    run_driver(tmp_path)
+    R, C, dx = spec.z.shape[0], spec.z.shape[1], spec.cellsize
+    z = po.lake_catchment_smooth(spec.lake_mask, spec.z)
+    o = po.Oracle(z, spec.h0, dx, dt=spec.dt, manning=spec.manning, INT=0.1, INF=0.05, LWL=0.2,
+                  EV_WL_min=0.01)
+    o.set_rain(spec.rain, spec.dtrain)
+    xs, ys, Q, dtoq = CASE_KW["sources"]
+    o.set_sources([po.source_index(R, C, 0.0, 0.0, dx, xs[0], ys[0])], Q, dtoq)
+    albedo, T, Rg = CASE_KW["evaporation"]
+    o.set_evaporation(po.evaporation(albedo, T, Rg, spec.dtrain))
+    dt, lkout = spec.dt, round(CASE_KW["dkout"] / spec.dt)
+    time, mstep, out0, tq = 0.0, 0, -1, -1
+    uua_l, vva_l = np.zeros((R, C)), np.zeros((R, C))
+    hist, wl = [], []
+    outdir = pathlib.Path(base) / "SW2D" / "output"
+    nfiles = 0
+    while time + dt <= spec.t_end:
+        out0 += 1
+        o.step(1)
+        time += dt
+        mstep += 1
+        time += dt
+        mstep += 1
+        if not (mstep % lkout == 0 or out0 == 0):
+            continue
+        tq += 1
+        tempo = "0" if out0 == 0 else str(int(round(mstep * dt / CASE_KW["dkout"])))
+        h = spec.h0 if tq == 0 else o.field("h")
+        rec = po.couple_bc(spec.lake_mask, h, vva_l, uua_l, dx, dx)
+        rec["time"] = 0.0 if tq == 0 else mstep * dt / CASE_KW["basetime"]
+        hist.append(rec)
+        uua_l, vva_l = o.field("uua"), o.field("vva")
+        depth = spec.h0 if out0 == 0 else o.field("h")
+        wl.append(water_level_line(R, C, dx, CASE_KW["probes"], depth))
+        exp = vtk_text(R, C, dx, depth, o.field("vv"), o.field("uu"), z)
+        got = (outdir / f"Results_{tempo}.vtk").read_text()
+        assert got == exp, f"Results_{tempo}.vtk differs"
+        nfiles += 1
+    assert nfiles == 5
+    assert (outdir / "WaterLevel.txt").read_text() == "".join(wl)
+    assert (outdir / f"TimeSimu_{tempo}.txt").exists()
+    got = (pathlib.Path(base) / "EFDC" / "input" / "QSER.INP").read_text()
+    assert got == qser_text(hist, CASE_KW["nlayers"], CASE_KW["basetime"])
+    assert any(v > 0 for r in hist[1:] for e in ("north", "south", "east", "west") for v in r[e]["q"])

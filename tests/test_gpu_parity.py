@@ -191,3 +191,21 @@ def test_fp32_precision_form(pkg, po, cases):
     assert dh.max() <= 1e-3, dh.max()
     b = g.balance()
     assert abs(b["error"]) <= 1e-6 * b["volume"]
+
+
+def test_autotune_keeps_results(pkg, po, cases):
+    """The measured tile height changes the schedule, never the numbers."""
+    R, C = 90, 140
+    z = dem(cases, R, C, 21)
+    h0 = np.zeros((R, C))
+    g, o = pair(pkg, po, z, h0, 10.0)
+    rain = np.array([80.0, 20.0, 0.0])
+    g.set_rain(rain, 100.0)
+    o.set_rain(rain, 100.0)
+    H = g.autotune(3)
+    assert 8 <= H <= 128
+    g.step(120)
+    o.step(120)
+    compare(g, o, tag=f"autotuned H={H}")
+    with pytest.raises(pkg.Sw2dError):
+        g.autotune(3)            # only before the first iteration
