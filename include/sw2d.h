@@ -20,6 +20,8 @@
 #ifndef SW2D_H
 #define SW2D_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -236,6 +238,135 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes);
  * (east-neighbour move; 0 for the last).  x, y, out hold n doubles (y may be
  * NULL where unused). */
 int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n);
+
+/* ------------------------------------------------------------------------
+ * Batches: many scenarios of any grid sizes advanced together, one kernel
+ * launch per iteration for all of them (groups of six iterations replay a
+ * captured HIP graph).  No counterpart in the reference, which runs one
+ * grid per process (SW2D-EFDC.cu:3682 main); each member keeps the
+ * single-grid semantics of sw2d_step exactly, so its results do not depend
+ * on what else is in the batch.
+ * ---------------------------------------------------------------------- */
+typedef struct sw2d_batch sw2d_batch;
+
+/* A batch on `device` owns one stream and a device memory pool of
+ * pool_bytes; every member's arrays come from the pool.  All members share
+ * `precision`.  tile_rows: rows per tile of members (0 = 32). */
+int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_rows, sw2d_batch **out);
+/* Destroys the members still in the batch, then the batch. */
+int sw2d_batch_destroy(sw2d_batch *b);
+const char *sw2d_batch_last_error(const sw2d_batch *b);
+
+/* Pool bytes a member with this grid and these params takes, with a
+ * hydrograph of hydro_cap records and (manning_map != 0) a Manning class
+ * map; tile_rows as for sw2d_batch_create. */
+size_t sw2d_batch_footprint(const sw2d_grid_desc *grid, const sw2d_params *params, int tile_rows,
+                            long hydro_cap, int manning_map);
+
+/* A handle whose arrays live in the batch's pool and whose work runs on the
+ * batch's stream.  All sw2d_* calls on handles apply to it; SW2D_E_NOMEM
+ * when the pool cannot hold it.  params->kernel must be fused and
+ * params->precision the batch's. */
+int sw2d_batch_new(sw2d_batch *b, const sw2d_grid_desc *grid, const double *z, const double *h0,
+                   const sw2d_params *params, sw2d_handle **out);
+
+/* Join the batch's step loop.  The member stops before iteration
+ * (current + max_iters) (max_iters < 0: no bound) and before the first
+ * iteration whose t + dt exceeds t_end (INFINITY: no bound), the loop
+ * condition of SW2D-EFDC.cu:4789.  While attached, sw2d_step on it is
+ * SW2D_E_STATE. */
+int sw2d_batch_attach(sw2d_batch *b, sw2d_handle *h, long max_iters, double t_end);
+int sw2d_batch_detach(sw2d_batch *b, sw2d_handle *h);
+
+/* Up to n_iters iterations of every attached member that has not finished;
+ * *running = members still running afterwards.  Synchronises. */
+int sw2d_batch_step(sw2d_batch *b, int n_iters, int *running);
+
+/* Progress of a member after the last sw2d_batch_step. */
+int sw2d_batch_status(sw2d_batch *b, const sw2d_handle *h, int *done, long *iters, double *time);
+
+/* Measured choice of the members' tile rows (all members must be at
+ * iteration 0): each candidate runs `iters` trial launches that write only
+ * spare buffers and scratch bookkeeping. */
+int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows);
+
+/* Pool use in bytes: now, the most so far, capacity. */
+int sw2d_batch_memory(sw2d_batch *b, size_t *in_use, size_t *peak, size_t *capacity);
+
+/* HIP-event timing of the step launches (as sw2d_timing / sw2d_kernel_time). */
+int sw2d_batch_timing(sw2d_batch *b, int enable);
+int sw2d_batch_kernel_time(sw2d_batch *b, double *ms, long *launches);
+
+/* ------------------------------------------------------------------------
+ * Serving: a request loop over one batch, run by a device thread the server
+ * owns.  Requests queue in arrival order and are admitted while the batch's
+ * memory pool can hold them (HBM-footprint admission); admitted scenarios
+ * are stepped together with those already running (continuous batching) and
+ * retire when they reach their end time, leaving their outlet hydrograph and
+ * volume ledger with the server.  No counterpart in the reference.
+ * ---------------------------------------------------------------------- */
+typedef struct sw2d_server sw2d_server;
+
+typedef struct sw2d_request {
+    sw2d_grid_desc grid;
+    const double *z;                 /* rows*cols bed elevations */
+    const double *h0;                /* rows*cols initial depths; NULL = dry */
+    sw2d_params params;              /* precision and kernel are the server's */
+    const double *rain_mm;           /* rain series (mm per dtrain), nrain values */
+    int nrain;
+    double dtrain;
+    int outlet_row, outlet_col;      /* free-outflow outlet; -1 = none */
+    const unsigned char *manning_class;  /* optional per-cell class map ... */
+    const double *manning_table;         /* ... and its nclass coefficients */
+    int nclass;
+    double t_end;                    /* simulated end time, s */
+    long max_iters;                  /* iteration bound (< 0 = none) */
+    long hydro_cap;                  /* hydrograph records kept (0 = enough for t_end at dt) */
+} sw2d_request;
+
+#define SW2D_REQ_QUEUED 0
+#define SW2D_REQ_RUNNING 1
+#define SW2D_REQ_DONE 2
+#define SW2D_REQ_FAILED (-1)
+
+typedef struct sw2d_result {
+    int state;                       /* SW2D_REQ_* */
+    double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+    long iterations;
+    double time;                     /* simulated time reached */
+    long hydro_n;                    /* hydrograph records held */
+    sw2d_balance_out balance;
+} sw2d_result;
+
+typedef struct sw2d_server_stats {
+    long submitted, completed, failed, running, queued;
+    double cell_updates;             /* rows*cols*iterations of completed requests */
+    double busy_seconds;             /* device-thread time spent stepping */
+    double uptime;                   /* seconds since create */
+    double p50_latency, p99_latency, max_latency;  /* t_done - t_submit, completed */
+    double p50_wait, p99_wait;       /* t_admit - t_submit, completed */
+    size_t pool_peak, pool_capacity;
+    long launches;
+    double kernel_ms;                /* HIP-event time of the step launches */
+} sw2d_server_stats;
+
+/* slice_iters: iterations per batch step between admissions (0 = 60). */
+int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_rows, int slice_iters,
+                       sw2d_server **out);
+/* Copies the request's arrays; returns its id. */
+int sw2d_server_submit(sw2d_server *s, const sw2d_request *req, long *id);
+int sw2d_server_poll(sw2d_server *s, long id, sw2d_result *out);
+/* (t, q) pairs of a finished request: 2*min(n, cap) doubles. */
+int sw2d_server_hydrograph(sw2d_server *s, long id, double *tq, long cap, long *n);
+/* Forget a finished request. */
+int sw2d_server_release(sw2d_server *s, long id);
+/* Wait until request id (id < 0: every submitted request) has finished or
+ * failed; SW2D_E_STATE on timeout. */
+int sw2d_server_wait(sw2d_server *s, long id, double timeout_s);
+int sw2d_server_stats_get(sw2d_server *s, sw2d_server_stats *out);
+const char *sw2d_server_last_error(const sw2d_server *s);
+/* Stops the device thread (queued requests are dropped) and frees everything. */
+int sw2d_server_destroy(sw2d_server *s);
 
 #ifdef __cplusplus
 }

@@ -56,6 +56,8 @@ def lib():
         L.orc_set_bc.argtypes = [ct.c_void_p, _IP, _IP, ct.c_int]
         L.orc_step.argtypes = [ct.c_void_p, ct.c_int]
         L.orc_round_state_f32.argtypes = [ct.c_void_p]
+        L.orc_set_state.argtypes = [ct.c_void_p, _DP, _DP, _DP]
+        L.orc_set_state.restype = None
         L.orc_run_until.argtypes = [ct.c_void_p, _D, ct.c_long]
         L.orc_run_until.restype = ct.c_long
         L.orc_time.argtypes = [ct.c_void_p]
@@ -214,6 +216,11 @@ class Oracle:
 
     def step(self, n=1):
         self._L.orc_step(self._s, n)
+
+    def set_state(self, h, um, vn):
+        """Replace (h, um, vn) and recompute the derived fields (test aid)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64).ravel() for x in (h, um, vn)]
+        self._L.orc_set_state(self._s, _dp(a[0]), _dp(a[1]), _dp(a[2]))
 
     def round_state_f32(self):
         self._L.orc_round_state_f32(self._s)

@@ -25,7 +25,8 @@
  * moving Gaussian storm) are hooks that stay off in reference mode; their
  * semantics are stated where they are implemented.
  *
- * Parity pinning of each function is done by tests/test_oracle_*.py.
+ * Each function is pinned by tests/test_oracle.py (closed forms, invariants,
+ * hand-computed branch values and the example case files of the reference).
  */
 #include <math.h>
 #include <stdint.h>
@@ -729,6 +730,17 @@ void orc_round_state_f32(orc_t *s)
         s->um[c] = (double)(float)s->um[c];
         s->vn[c] = (double)(float)s->vn[c];
     }
+    orc_derived(s);
+}
+
+/* test aid: replace (h, um, vn) of the R*C real cells and recompute the
+ * derived fields, as the end of an iteration does (hm_hn .. forward) */
+void orc_set_state(orc_t *s, const double *h, const double *um, const double *vn)
+{
+    const size_t RC = (size_t)s->R * s->C;
+    memcpy(s->h, h, sizeof(double) * RC);
+    memcpy(s->um, um, sizeof(double) * RC);
+    memcpy(s->vn, vn, sizeof(double) * RC);
     orc_derived(s);
 }
 

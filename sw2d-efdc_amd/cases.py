@@ -16,6 +16,16 @@ import numpy as np
 BASE_SEED = 20261202
 
 
+try:  # scipy's pocketfft gives numpy's values bit for bit, on several threads
+    import scipy.fft as _sfft
+
+    def _irfft2(a, s):
+        return _sfft.irfft2(a, s=s, workers=-1)
+except ImportError:  # pragma: no cover
+    def _irfft2(a, s):
+        return np.fft.irfft2(a, s=s)
+
+
 def fbm(rows, cols, hurst, amplitude, seed):
     rng = np.random.default_rng(seed)
     ky = np.fft.fftfreq(rows)[:, None]
@@ -25,7 +35,7 @@ def fbm(rows, cols, hurst, amplitude, seed):
     amp = k ** (-(hurst + 1.0))
     amp[0, 0] = 0.0
     noise = rng.standard_normal(amp.shape) + 1j * rng.standard_normal(amp.shape)
-    surf = np.fft.irfft2(noise * amp, s=(rows, cols))
+    surf = _irfft2(noise * amp, s=(rows, cols))
     span = surf.max() - surf.min()
     if span > 0:
         surf = (surf - surf.min()) / span * amplitude
@@ -133,6 +143,69 @@ def config2(rows=4096, cols=4096, seed=BASE_SEED + 2, t_end=1800.0):
     h0 = np.where(res, np.maximum(level - z, 0.0), 0.0)
     return CaseSpec("config2", z, h0, dx, dt=0.5, manning=0.035, t_end=t_end, adaptive=True,
                     courant=0.5)
+
+
+def block_rain(intensity_mm_h, duration_min, n_after=3):
+    """rain.dat of a block storm: one interval of the storm's length holding its
+    depth in mm, then dry intervals (config 0 writes 50 mm/h for 30 min as
+    dtrain = 1800 s, values 25, 0, 0, 0)."""
+    dtrain = duration_min * 60.0
+    return np.array([intensity_mm_h * duration_min / 60.0] + [0.0] * n_after), dtrain
+
+
+def ensemble_scenario(k, base_seed=BASE_SEED + 3, side=None, t_end=3600.0, field_cache=None):
+    """Scenario k of the batched ensemble: square grid, side = round(128 * 8^U)
+    (log-uniform 128..1024) unless given, dx = 10 m; DEM as config 0 with slope
+    U(0.2 %, 3 %) and fBm amplitude U(0.5, 5) m; n ~ U(0.02, 0.1); block rain
+    U(10, 120) mm/h for U(15, 90) min; outlet at the lowest south-edge cell;
+    fixed dt = 0.5 s; 1 h.  Seeded by base_seed + k.  field_cache maps a side
+    to a list of unit-amplitude fBm fields to draw from instead of synthesising
+    one per scenario (the serving generator uses it: thousands of requests)."""
+    rng = np.random.default_rng(base_seed + k)
+    if side is None:
+        side = int(round(128 * 8.0 ** rng.random()))
+    dx = 10.0
+    slope = rng.uniform(0.002, 0.03)
+    amp = rng.uniform(0.5, 5.0)
+    n = rng.uniform(0.02, 0.1)
+    inten = rng.uniform(10.0, 120.0)
+    dur = rng.uniform(15.0, 90.0)
+    if field_cache is not None:
+        pool = field_cache[side]
+        f = pool[int(rng.integers(len(pool)))] * amp
+    else:
+        f = fbm(side, side, 0.7, amp, base_seed + k)
+    r = np.arange(side, dtype=np.float64)[:, None]
+    z = slope * dx * (side - 1 - r) + f
+    rain, dtrain = block_rain(inten, dur, n_after=max(3, int(np.ceil(t_end / (dur * 60.0))) + 1))
+    spec = CaseSpec(f"ens{k}", z, np.zeros((side, side)), dx, dt=0.5, manning=n, t_end=t_end,
+                    rain=rain, dtrain=dtrain)
+    spec.outlet = (side - 1, int(np.argmin(z[side - 1])))
+    spec.extra = dict(slope=slope, amp=amp, intensity=inten, duration=dur)
+    return spec
+
+
+SERVE_SIDES = (256, 512, 1024, 2048, 4096)
+SERVE_PROBS = (0.4, 0.3, 0.15, 0.1, 0.05)
+
+
+def serve_requests(n, rate=20.0, seed=BASE_SEED + 4, t_end=7200.0, fields_per_side=2,
+                   sides=SERVE_SIDES, probs=SERVE_PROBS):
+    """The serving stream: Poisson arrivals at `rate` per second, grid side drawn
+    from `sides` with `probs`, scenarios as ensemble_scenario (2 h storms).
+    Returns (arrival offsets in s, specs).  DEM relief comes from a cache of
+    `fields_per_side` seeded unit fBm fields per side, scaled per request."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    chosen = rng.choice(len(sides), size=n, p=np.asarray(probs) / np.sum(probs))
+    cache = {}
+    for i in sorted(set(chosen.tolist())):
+        s = sides[i]
+        cache[s] = [fbm(s, s, 0.7, 1.0, seed + 1000 * s + j) for j in range(fields_per_side)]
+    specs = [ensemble_scenario(k, base_seed=seed + 17, side=sides[chosen[k]], t_end=t_end,
+                               field_cache=cache) for k in range(n)]
+    return arrivals, specs
 
 
 def apply(case_obj, spec):

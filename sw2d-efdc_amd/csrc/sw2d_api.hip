@@ -7,53 +7,11 @@
 #include <vector>
 
 #include "sw2d_common.hpp"
+#include "sw2d_ops.hpp"
 
 using namespace sw2d;
 
-namespace {
-
-thread_local std::string g_create_error;
-
-int fail(sw2d_handle *h, int code, const std::string &msg)
-{
-    if (h) h->err = msg;
-    else g_create_error = msg;
-    return code;
-}
-
-int hip_fail(sw2d_handle *h, hipError_t e, const char *what)
-{
-    std::string m = std::string(what) + ": " + hipGetErrorString(e);
-    return fail(h, SW2D_E_HIP, m);
-}
-
-#define HIPCHK(h, expr)                                   \
-    do {                                                  \
-        hipError_t e_ = (expr);                           \
-        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
-    } while (0)
-
-template <typename T>
-int dalloc(sw2d_handle *h, T **p, size_t count)
-{
-    void *q = nullptr;
-    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
-    hipError_t e = hipMalloc(&q, bytes);
-    if (e != hipSuccess) return fail(h, SW2D_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemsetAsync(q, 0, bytes, h->stream);
-    if (e != hipSuccess) return hip_fail(h, e, "hipMemsetAsync");
-    h->allocs.push_back(q);
-    *p = (T *)q;
-    return SW2D_OK;
-}
-
-void dfree(sw2d_handle *h, void *p)
-{
-    if (!p) return;
-    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
-    if (it != h->allocs.end()) h->allocs.erase(it);
-    (void)hipFree(p);
-}
+namespace sw2d {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
@@ -98,14 +56,103 @@ StepCommon make_common(sw2d_handle *h)
 
 bool fused(const sw2d_handle *h) { return h->prm.kernel == SW2D_KERNEL_FUSED; }
 
+int fused_blocks(const sw2d_handle *h) { return h->ntiles + 1 + (h->nbc + 63) / 64; }
+
+// the fused kernel's descriptor of a stand-alone handle (phase 0 = iteration 0)
+Scenario make_scenario(sw2d_handle *h)
+{
+    Scenario d;
+    memset(&d, 0, sizeof d);
+    d.s = make_common(h);
+    for (int b = 0; b < 2; b++) {
+        d.st[b][0] = h->h[b];
+        d.st[b][1] = h->um[b];
+        d.st[b][2] = h->vn[b];
+    }
+    d.z = (const double *)h->z;
+    d.cls = h->cls;
+    d.P = h->P; d.W = h->W; d.H = h->H;
+    d.nstrips = h->nstrips; d.ntiles = h->ntiles;
+    d.nblocks = fused_blocks(h);
+    d.part = h->part;
+    d.s.nparts = h->nparts;
+    d.phase0 = 0;
+    d.k_end = 0x7fffffffffffffffLL;
+    d.t_end = INFINITY;
+    d.status = nullptr;
+    return d;
+}
+
+int upload_desc(sw2d_handle *h)
+{
+    if (!h->desc_dirty) return SW2D_OK;
+    const Scenario d = make_scenario(h);
+    HIPCHK(h, hipMemcpyAsync(h->dscen, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    h->desc_dirty = false;
+    return SW2D_OK;
+}
+
+hipError_t launch_one(sw2d_handle *h, const Scenario *scen, int phase)
+{
+    return h->esize == 8
+               ? launch_fused_f64(h->stream, scen, nullptr, fused_blocks(h), phase)
+               : launch_fused_f32(h->stream, scen, nullptr, fused_blocks(h), phase);
+}
+
+// the graph of iterations k .. k+5 (k % 6 == 0): six launches, phases 0..5
+int ensure_graph(sw2d_handle *h)
+{
+    if (h->gexec) return SW2D_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int ph = 0; ph < 6 && e == hipSuccess; ph++) e = launch_one(h, h->dscen, ph);
+    hipError_t e2 = hipStreamEndCapture(h->stream, &g);
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(h, e, "graph capture launch"); }
+    if (e2 != hipSuccess) return hip_fail(h, e2, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { h->gexec = nullptr; return hip_fail(h, e, "hipGraphInstantiate"); }
+    return SW2D_OK;
+}
+
+// host mirror after m iterations
+void advance(sw2d_handle *h, long m)
+{
+    for (long i = 0; i < m; i++) {
+        if (!h->prm.adaptive) {
+            h->host_time = h->host_time + h->prm.dt;
+            h->host_time = h->host_time + h->prm.dt;
+        }
+    }
+    h->step_index += m;
+    h->host_mstep += 2 * m;
+    h->cur = (int)(h->step_index & 1);
+}
+
+int flush_events(sw2d_handle *h)
+{
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    for (auto &ev : h->events) {
+        float ms = 0;
+        HIPCHK(h, hipEventElapsedTime(&ms, ev.first, ev.second));
+        h->kernel_ms += ms;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    h->events.clear();
+    return SW2D_OK;
+}
+
 // Default tile height of the fused kernel (every tile is one wave): 16 rows,
 // lowered towards 8 while the grid has fewer tiles than resident wave slots
 // (occupancy x CUs).  sw2d_autotune replaces it by a measured choice.
 int choose_tile_rows(const sw2d_handle *h)
 {
     if (h->prm.tile_rows > 0) return std::min(h->prm.tile_rows, 512);
-    int per_cu = h->esize == 8 ? fused_waves_per_cu_f64(h->prm.adaptive)
-                               : fused_waves_per_cu_f32(h->prm.adaptive);
+    int per_cu = h->esize == 8 ? fused_waves_per_cu_f64()
+                               : fused_waves_per_cu_f32();
     hipDeviceProp_t prop;
     int cus = 0;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess) cus = prop.multiProcessorCount;
@@ -118,6 +165,7 @@ int choose_tile_rows(const sw2d_handle *h)
 
 void set_tiles(sw2d_handle *h, int H)
 {
+    if (H != h->H) changed(h);
     h->H = H;
     h->nstrips = (h->C + h->W - 1) / h->W;
     h->nchunks = (h->R + H - 1) / H;
@@ -167,8 +215,12 @@ const char *sw2d_last_error(const sw2d_handle *h)
     return h ? h->err.c_str() : g_create_error.c_str();
 }
 
-int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
-                const sw2d_params *params, sw2d_handle **out)
+}  // extern "C"
+
+namespace sw2d {
+
+int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
+                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out)
 {
     if (!out) return fail(nullptr, SW2D_E_ARG, "out is NULL");
     *out = nullptr;
@@ -190,11 +242,20 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
         return fail(nullptr, SW2D_E_NODEV, "no HIP device");
     if (params->device < 0 || params->device >= ndev) return fail(nullptr, SW2D_E_NODEV, "bad device ordinal");
 
+    if (batch) {
+        if (params->kernel != SW2D_KERNEL_FUSED)
+            return fail(nullptr, SW2D_E_UNSUPPORTED, "batch members use the fused kernel");
+        if (params->precision != batch->precision)
+            return fail(nullptr, SW2D_E_ARG, "member precision differs from the batch's");
+    }
+
     sw2d_handle *h = new sw2d_handle();
+    h->batch = batch;
     h->grid = *grid;
     h->prm = *params;
     h->R = grid->rows; h->C = grid->cols;
-    h->device = params->device;
+    h->device = batch ? batch->device : params->device;
+    h->prm.device = h->device;
     h->esize = params->precision == SW2D_FP64 ? 8 : 4;
     int rc = SW2D_OK;
     auto bail = [&](int code) {
@@ -204,8 +265,12 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
     };
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) { hip_fail(h, e, "hipSetDevice"); return bail(SW2D_E_HIP); }
-    e = hipStreamCreate(&h->stream);  // blocking stream: ordered with hipMemcpy on the null stream
-    if (e != hipSuccess) { hip_fail(h, e, "hipStreamCreate"); return bail(SW2D_E_HIP); }
+    if (batch) {
+        h->stream = batch->stream;
+    } else {
+        e = hipStreamCreate(&h->stream);  // blocking stream: ordered with hipMemcpy on the null stream
+        if (e != hipSuccess) { hip_fail(h, e, "hipStreamCreate"); return bail(SW2D_E_HIP); }
+    }
 
     const int R = h->R, C = h->C;
     const size_t RC = (size_t)R * C;
@@ -221,7 +286,7 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
         h->P = round_up(C, 16);
         const size_t n = (size_t)R * h->P;
         h->W = 58;
-        set_tiles(h, choose_tile_rows(h));
+        set_tiles(h, (batch && batch->H > 0 && params->tile_rows == 0) ? batch->H : choose_tile_rows(h));
         h->nparts = h->ntiles;
         void **bufs[] = {&h->h[0], &h->h[1], &h->um[0], &h->um[1], &h->vn[0], &h->vn[1]};
         for (void **b : bufs) {
@@ -229,7 +294,7 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
             else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
         }
         if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
-        if ((rc = dalloc(h, &h->scratch, RC))) return bail(rc);
+        if ((rc = dalloc(h, &h->dscen, 1))) return bail(rc);
         // pitched images: inactive cells carry NaN in z, which stays FP64
         std::vector<double> zi(n, 0.0);
         for (int r = 0; r < R; r++)
@@ -285,21 +350,42 @@ int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
     e = !fused(h) ? launch_plain_init(h) : (h->esize == 8 ? launch_init_f64(h) : launch_init_f32(h));
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) { hip_fail(h, e, "init kernel"); return bail(SW2D_E_HIP); }
+    if (batch) batch->owned.push_back(h);
     *out = h;
     return SW2D_OK;
+}
+
+}  // namespace sw2d
+
+extern "C" {
+
+int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
+                const sw2d_params *params, sw2d_handle **out)
+{
+    return create_impl(grid, z, h0, params, nullptr, out);
 }
 
 int sw2d_destroy(sw2d_handle *h)
 {
     if (!h) return SW2D_OK;
+    if (h->attached) (void)sw2d_batch_detach(h->batch, h);
+    if (h->batch) {
+        auto &own = h->batch->owned;
+        auto it = std::find(own.begin(), own.end(), h);
+        if (it != own.end()) own.erase(it);
+    }
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (auto &ev : h->events) {
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);
     }
-    for (void *p : h->allocs) (void)hipFree(p);
+    drop_graph(h);
+    for (void *p : h->allocs) {
+        if (h->batch) (void)h->batch->pool.release(p);
+        else (void)hipFree(p);
+    }
     h->allocs.clear();
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    if (h->stream && !h->batch) (void)hipStreamDestroy(h->stream);
     delete h;
     return SW2D_OK;
 }
@@ -317,6 +403,7 @@ int sw2d_set_rain_series(sw2d_handle *h, const double *rain_mm, int n, double dt
     if (n > 0) HIPCHK(h, hipMemcpy(h->rain, rain_mm, sizeof(double) * n, hipMemcpyHostToDevice));
     h->nrain = n;
     h->dtrain = dtrain;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -353,6 +440,7 @@ int sw2d_set_sources(sw2d_handle *h, const double *x, const double *y, int nst, 
     if ((size_t)nq * nst > 0)
         HIPCHK(h, hipMemcpy(h->Q, Q, sizeof(double) * nq * nst, hipMemcpyHostToDevice));
     h->nst = nst; h->nq = nq; h->dtoq = dtoq;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -368,6 +456,7 @@ int sw2d_set_evaporation(sw2d_handle *h, const double *evapo_mm, int n)
     if (n > 0) HIPCHK(h, hipMemcpy(h->evapo, evapo_mm, sizeof(double) * n, hipMemcpyHostToDevice));
     h->nevapo = n;
     h->evap_on = 1;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -395,6 +484,7 @@ int sw2d_set_manning_map(sw2d_handle *h, const unsigned char *cls, const double 
         HIPCHK(h, hipMemcpy(h->pncell, nc.data(), nc.size() * 8, hipMemcpyHostToDevice));
     }
     h->ncls_on = 1;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -405,6 +495,7 @@ int sw2d_set_storm(sw2d_handle *h, const sw2d_storm *storm)
     if (fused(h) && h->H > 512) return fail(h, SW2D_E_ARG, "tile too tall for the storm table");
     h->storm = *storm;
     h->storm_on = 1;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -419,6 +510,7 @@ int sw2d_set_outlet(sw2d_handle *h, int row, int col)
     else if (col == h->C - 1) edge = 4;
     if (!edge) return fail(h, SW2D_E_ARG, "outlet must be on a domain edge");
     h->out_r = row; h->out_c = col; h->out_edge = edge;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -432,6 +524,7 @@ int sw2d_set_hydrograph(sw2d_handle *h, long cap)
     int rc = dalloc(h, &h->hydro, 2 * (size_t)cap);
     if (rc) return rc;
     h->hydro_cap = cap;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -469,6 +562,7 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask)
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     h->nbc = n;
+    changed(h);
     return SW2D_OK;
 }
 
@@ -477,6 +571,7 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     if (check_idle(h)) return SW2D_E_ARG;
     if (iters < 1) return fail(h, SW2D_E_ARG, "iters must be positive");
     if (h->step_index != 0) return fail(h, SW2D_E_STATE, "autotune runs before the first iteration");
+    if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tuned by sw2d_batch_autotune");
     if (!fused(h)) {
         if (tile_rows) *tile_rows = 0;
         return SW2D_OK;
@@ -490,9 +585,11 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     Partial *part_s = nullptr;
     Ledger *led_s = nullptr;
     double *bc_s = nullptr;
+    Scenario *desc_s = nullptr;
     int rc;
     if ((rc = dalloc(h, &clk_s, 3)) || (rc = dalloc(h, &part_s, 2 * (size_t)max_tiles)) ||
-        (rc = dalloc(h, &led_s, 1)) || (rc = dalloc(h, &bc_s, std::max(h->nbc, 1))))
+        (rc = dalloc(h, &led_s, 1)) || (rc = dalloc(h, &bc_s, std::max(h->nbc, 1))) ||
+        (rc = dalloc(h, &desc_s, 1)))
         return rc;
     hipEvent_t e0, e1;
     HIPCHK(h, hipEventCreate(&e0));
@@ -503,23 +600,23 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     for (int H : cand) {
         if (H > h->R && H != cand[0]) break;
         set_tiles(h, H);
-        StepCommon sc = make_common(h);
+        Scenario d = make_scenario(h);
+        d.s.clk = clk_s;
+        d.part = part_s;
+        d.s.nparts = h->ntiles;
+        d.s.ledger = led_s;
+        d.s.hydro = nullptr;
+        d.s.bc_acc = bc_s;
         HIPCHK(h, hipMemcpyAsync(clk_s, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, h->stream));
-        sc.clk = clk_s;
-        sc.part_out = part_s;
-        sc.part_in = part_s + max_tiles;
-        sc.nparts = h->ntiles;
-        sc.ledger = led_s;
-        sc.hydro = nullptr;
-        sc.bc_acc = bc_s;
-        auto launch = [&]() {
-            return h->esize == 8 ? launch_fused_f64(h, sc) : launch_fused_f32(h, sc);
-        };
-        hipError_t e = launch();  // warm-up of this shape
+        HIPCHK(h, hipMemcpyAsync(desc_s, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
+        HIPCHK(h, hipStreamSynchronize(h->stream));
+        // every trial launch is iteration 0 again: it reads slot 0 of the
+        // scratch clock and buffer 0, and writes slot 1 and buffer 1
+        hipError_t e = launch_one(h, desc_s, 0);  // warm-up of this shape
         if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
         HIPCHK(h, hipEventRecord(e0, h->stream));
         for (int i = 0; i < iters; i++)
-            if ((e = launch()) != hipSuccess) return hip_fail(h, e, "autotune launch");
+            if ((e = launch_one(h, desc_s, 0)) != hipSuccess) return hip_fail(h, e, "autotune launch");
         HIPCHK(h, hipEventRecord(e1, h->stream));
         HIPCHK(h, hipEventSynchronize(e1));
         float ms = 0;
@@ -528,7 +625,7 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s);
+    dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
     set_tiles(h, best);
     if (h->ntiles > h->nparts) {
         dfree(h, h->part);
@@ -536,6 +633,7 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
         if ((rc = dalloc(h, &h->part, 2 * (size_t)h->ntiles))) return rc;
     }
     h->nparts = h->ntiles;
+    changed(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (tile_rows) *tile_rows = best;
     return SW2D_OK;
@@ -545,40 +643,59 @@ int sw2d_step(sw2d_handle *h, int n_iters)
 {
     if (check_idle(h)) return SW2D_E_ARG;
     if (n_iters < 0) return fail(h, SW2D_E_ARG, "negative iteration count");
-    for (int i = 0; i < n_iters; i++) {
-        const StepCommon sc = make_common(h);
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (h->timing) {
-            HIPCHK(h, hipEventCreate(&e0));
-            HIPCHK(h, hipEventCreate(&e1));
-            HIPCHK(h, hipEventRecord(e0, h->stream));
-        }
-        hipError_t e = !fused(h) ? launch_plain(h, sc)
-                       : (h->esize == 8 ? launch_fused_f64(h, sc) : launch_fused_f32(h, sc));
-        if (e != hipSuccess) return hip_fail(h, e, "step launch");
-        if (h->timing) {
-            HIPCHK(h, hipEventRecord(e1, h->stream));
-            h->events.emplace_back(e0, e1);
-            if (h->events.size() >= 1024) {
-                HIPCHK(h, hipStreamSynchronize(h->stream));
-                for (auto &ev : h->events) {
-                    float ms = 0;
-                    HIPCHK(h, hipEventElapsedTime(&ms, ev.first, ev.second));
-                    h->kernel_ms += ms;
-                    (void)hipEventDestroy(ev.first);
-                    (void)hipEventDestroy(ev.second);
-                }
-                h->events.clear();
+    if (h->attached) return fail(h, SW2D_E_STATE, "an attached batch member is stepped by sw2d_batch_step");
+    if (!fused(h)) {
+        for (int i = 0; i < n_iters; i++) {
+            const StepCommon sc = make_common(h);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (h->timing) {
+                HIPCHK(h, hipEventCreate(&e0));
+                HIPCHK(h, hipEventCreate(&e1));
+                HIPCHK(h, hipEventRecord(e0, h->stream));
             }
-            h->launches++;
+            hipError_t e = launch_plain(h, sc);
+            if (e != hipSuccess) return hip_fail(h, e, "step launch");
+            if (h->timing) {
+                HIPCHK(h, hipEventRecord(e1, h->stream));
+                h->events.emplace_back(e0, e1);
+                h->launches++;
+                if (h->events.size() >= 1024 && flush_events(h)) return SW2D_E_HIP;
+            }
+            advance(h, 1);
         }
-        if (fused(h)) h->cur = 1 - h->cur;
-        h->step_index++;
-        if (!h->prm.adaptive) {
-            h->host_time = h->host_time + h->prm.dt;
-            h->host_time = h->host_time + h->prm.dt;
+        return SW2D_OK;
+    }
+    int rc = upload_desc(h);
+    if (rc) return rc;
+    long n = n_iters;
+    while (n > 0) {
+        // whole groups of six from phase 0 replay the captured graph
+        const bool group = h->graph_on && (h->step_index % 6 == 0) && n >= 6;
+        if (group && (rc = ensure_graph(h))) return rc;
+        const long m = group ? 6 * (n / 6) : 1;
+        const long reps = group ? m / 6 : 1;
+        for (long r = 0; r < reps; r++) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (h->timing) {
+                HIPCHK(h, hipEventCreate(&e0));
+                HIPCHK(h, hipEventCreate(&e1));
+                HIPCHK(h, hipEventRecord(e0, h->stream));
+            }
+            if (group) {
+                HIPCHK(h, hipGraphLaunch(h->gexec, h->stream));
+            } else {
+                hipError_t e = launch_one(h, h->dscen, (int)(h->step_index % 6));
+                if (e != hipSuccess) return hip_fail(h, e, "step launch");
+            }
+            if (h->timing) {
+                HIPCHK(h, hipEventRecord(e1, h->stream));
+                h->events.emplace_back(e0, e1);
+                h->launches += group ? 6 : 1;
+                if (h->events.size() >= 1024 && flush_events(h)) return SW2D_E_HIP;
+            }
         }
-        h->host_mstep += 2;
+        advance(h, m);
+        n -= m;
     }
     return SW2D_OK;
 }
@@ -659,6 +776,9 @@ int sw2d_get_field(sw2d_handle *h, int field, double *out)
     const size_t RC = (size_t)h->R * h->C;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (fused(h)) {
+        // derived fields go through an R*C FP64 buffer made on first use
+        int rc;
+        if (!h->scratch && (rc = dalloc(h, &h->scratch, RC))) return rc;
         hipError_t e = h->esize == 8 ? launch_derive_f64(h, field, h->scratch)
                                      : launch_derive_f32(h, field, h->scratch);
         if (e != hipSuccess) return hip_fail(h, e, "derive");
@@ -761,6 +881,8 @@ int sw2d_kernel_time(sw2d_handle *h, double *ms, long *launches)
         (void)hipEventDestroy(ev.second);
     }
     h->events.clear();
+    // graph replays are timed as a whole: ms / launches is then the mean
+    // launch-to-launch time of the six kernels
     if (ms) *ms = h->kernel_ms;
     if (launches) *launches = h->launches;
     return SW2D_OK;
@@ -771,7 +893,7 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
     if (check_idle(h) || !bytes) return SW2D_E_ARG;
     long long b = 0;
     const size_t RC = (size_t)h->R * h->C;
-    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (long long)RC * 8 + (h->cls ? (long long)h->R * h->P : 0);
+    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? (long long)h->R * h->P : 0);
     else b += (long long)16 * h->NA * 8;
     b += (long long)2 * h->nparts * sizeof(Partial);
     *bytes = b;

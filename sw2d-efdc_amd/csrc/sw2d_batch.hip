@@ -1,0 +1,469 @@
+// sw2d_batch.hip -- batches: many scenarios stepped by one launch per
+// iteration (C ABI in include/sw2d.h).
+//
+// Each member is an ordinary handle whose arrays come from the batch's memory
+// pool and whose work runs on the batch's stream.  A launch covers the blocks
+// of every attached member: block b of the grid reads entry b of a tile table
+// (scenario slot, block within the scenario) and the scenario's descriptor,
+// so members of different sizes, forcing and time steps share one grid.
+// Members join at any launch: a member's iteration k runs at launch phase
+// (phase0 + k) mod 6, and each member stops by itself (a done flag its own
+// bookkeeping wave sets) when its iteration or time bound is reached.
+#include <math.h>
+#include <stdio.h>
+
+#include "sw2d_common.hpp"
+#include "sw2d_ops.hpp"
+
+using namespace sw2d;
+
+namespace {
+
+int bfail(sw2d_batch *b, int code, const std::string &msg)
+{
+    if (b) b->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+int bhip(sw2d_batch *b, hipError_t e, const char *what)
+{
+    return bfail(b, SW2D_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define BCHK(b, expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return bhip(b, e_, #expr); \
+    } while (0)
+
+void bdrop_graph(sw2d_batch *b)
+{
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    b->gexec = nullptr;
+}
+
+hipError_t blaunch(sw2d_batch *b, const Scenario *scen, const TileRef *tiles, int nblocks, int phase)
+{
+    return b->esize == 8 ? launch_fused_f64(b->stream, scen, tiles, nblocks, phase)
+                         : launch_fused_f32(b->stream, scen, tiles, nblocks, phase);
+}
+
+// descriptors and tile table of the attached members
+int rebuild(sw2d_batch *b)
+{
+    if (!b->dirty) return SW2D_OK;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    std::vector<Scenario> desc;
+    std::vector<TileRef> tiles;
+    int top = 0;
+    for (sw2d_handle *h : b->members) top = std::max(top, h->member_slot + 1);
+    desc.resize(top);
+    memset(desc.data(), 0, sizeof(Scenario) * desc.size());
+    for (sw2d_handle *h : b->members) {
+        Scenario d = make_scenario(h);
+        d.phase0 = h->phase0;
+        d.k_end = h->k_end;
+        d.t_end = h->t_end;
+        d.status = b->dstatus + h->member_slot;
+        desc[h->member_slot] = d;
+        for (int i = 0; i < d.nblocks; i++) tiles.push_back(TileRef{h->member_slot, i});
+        h->desc_dirty = true;  // its own descriptor is refreshed when it steps alone again
+    }
+    if ((int)tiles.size() > b->tiles_cap) {
+        bdrop_graph(b);
+        if (b->dtiles) (void)hipFree(b->dtiles);
+        b->dtiles = nullptr;
+        b->tiles_cap = 0;
+        const int cap = std::max<int>(1024, (int)tiles.size() * 2);
+        BCHK(b, hipMalloc(&b->dtiles, sizeof(TileRef) * cap));
+        b->tiles_cap = cap;
+    }
+    if (top > 0)
+        BCHK(b, hipMemcpy(b->dscen, desc.data(), sizeof(Scenario) * top, hipMemcpyHostToDevice));
+    if (!tiles.empty())
+        BCHK(b, hipMemcpy(b->dtiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice));
+    if ((int)tiles.size() != b->nblocks) bdrop_graph(b);
+    b->nblocks = (int)tiles.size();
+    b->dirty = false;
+    return SW2D_OK;
+}
+
+int ensure_bgraph(sw2d_batch *b)
+{
+    if (b->gexec) return SW2D_OK;
+    hipGraph_t g = nullptr;
+    BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int ph = 0; ph < 6 && e == hipSuccess; ph++) e = blaunch(b, b->dscen, b->dtiles, b->nblocks, ph);
+    hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+    if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return bhip(b, e, "graph capture launch"); }
+    if (e2 != hipSuccess) return bhip(b, e2, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { b->gexec = nullptr; return bhip(b, e, "hipGraphInstantiate"); }
+    return SW2D_OK;
+}
+
+// members' host mirrors from the device status records
+int refresh(sw2d_batch *b, int *running)
+{
+    int top = 0;
+    for (sw2d_handle *h : b->members) top = std::max(top, h->member_slot + 1);
+    if (top > 0) {
+        BCHK(b, hipMemcpyAsync(b->hstatus, b->dstatus, sizeof(Status) * top, hipMemcpyDeviceToHost, b->stream));
+    }
+    BCHK(b, hipStreamSynchronize(b->stream));
+    int run = 0;
+    for (sw2d_handle *h : b->members) {
+        const Status &s = b->hstatus[h->member_slot];
+        const long long dk = s.k - h->step_index;
+        h->step_index = s.k;
+        h->host_mstep += 2 * dk;
+        h->host_time = s.t;
+        h->cur = (int)(h->step_index & 1);
+        if (!(s.done || s.fin)) run++;
+    }
+    if (running) *running = run;
+    return SW2D_OK;
+}
+
+int flush_bevents(sw2d_batch *b)
+{
+    BCHK(b, hipStreamSynchronize(b->stream));
+    for (auto &ev : b->events) {
+        float ms = 0;
+        BCHK(b, hipEventElapsedTime(&ms, ev.first, ev.second));
+        b->kernel_ms += ms;
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    b->events.clear();
+    return SW2D_OK;
+}
+
+size_t pool_bytes_of(size_t bytes) { return Pool::rounded(bytes); }
+
+}  // namespace
+
+extern "C" {
+
+int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_rows, sw2d_batch **out)
+{
+    if (!out) return bfail(nullptr, SW2D_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (precision != SW2D_FP64 && precision != SW2D_FP32) return bfail(nullptr, SW2D_E_ARG, "unknown precision");
+    if (tile_rows < 0 || tile_rows > 512) return bfail(nullptr, SW2D_E_ARG, "tile_rows must be in [0, 512]");
+    if (pool_bytes < (1u << 20)) return bfail(nullptr, SW2D_E_ARG, "pool must be at least 1 MiB");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bfail(nullptr, SW2D_E_NODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return bfail(nullptr, SW2D_E_NODEV, "bad device ordinal");
+    sw2d_batch *b = new sw2d_batch();
+    b->device = device;
+    b->precision = precision;
+    b->esize = precision == SW2D_FP64 ? 8 : 4;
+    b->H = tile_rows > 0 ? tile_rows : 32;
+    auto bail = [&](hipError_t e, const char *what) {
+        bhip(b, e, what);
+        g_create_error = b->err;
+        sw2d_batch_destroy(b);
+        return SW2D_E_HIP;
+    };
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return bail(e, "hipSetDevice");
+    if ((e = hipStreamCreate(&b->stream)) != hipSuccess) return bail(e, "hipStreamCreate");
+    e = hipMalloc(&b->pool_base, pool_bytes);
+    if (e != hipSuccess) {
+        g_create_error = std::string("pool hipMalloc: ") + hipGetErrorString(e);
+        sw2d_batch_destroy(b);
+        return SW2D_E_NOMEM;
+    }
+    b->pool.init(b->pool_base, pool_bytes);
+    if ((e = hipMalloc(&b->dscen, sizeof(Scenario) * kMaxMembers)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipMalloc(&b->dstatus, sizeof(Status) * kMaxMembers)) != hipSuccess) return bail(e, "hipMalloc");
+    if ((e = hipHostMalloc(&b->hstatus, sizeof(Status) * kMaxMembers, 0)) != hipSuccess) return bail(e, "hipHostMalloc");
+    for (int i = kMaxMembers - 1; i >= 0; i--) b->free_slots.push_back(i);
+    *out = b;
+    return SW2D_OK;
+}
+
+int sw2d_batch_destroy(sw2d_batch *b)
+{
+    if (!b) return SW2D_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    // members still attached or merely created in the pool go with the batch
+    std::vector<sw2d_handle *> ms = b->owned;
+    for (sw2d_handle *h : ms) (void)sw2d_destroy(h);
+    for (auto &ev : b->events) {
+        (void)hipEventDestroy(ev.first);
+        (void)hipEventDestroy(ev.second);
+    }
+    bdrop_graph(b);
+    if (b->dtiles) (void)hipFree(b->dtiles);
+    if (b->dscen) (void)hipFree(b->dscen);
+    if (b->dstatus) (void)hipFree(b->dstatus);
+    if (b->hstatus) (void)hipHostFree(b->hstatus);
+    if (b->pool_base) (void)hipFree(b->pool_base);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SW2D_OK;
+}
+
+const char *sw2d_batch_last_error(const sw2d_batch *b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+size_t sw2d_batch_footprint(const sw2d_grid_desc *grid, const sw2d_params *params, int tile_rows,
+                            long hydro_cap, int manning_map)
+{
+    if (!grid || !params || grid->rows < 2 || grid->cols < 2) return 0;
+    const size_t R = grid->rows, C = grid->cols;
+    const size_t P = (size_t)round_up((int)C, 16);
+    const size_t es = params->precision == SW2D_FP64 ? 8 : 4;
+    const int H = params->tile_rows > 0 ? params->tile_rows : (tile_rows > 0 ? tile_rows : 32);
+    const size_t ntiles = ((C + 57) / 58) * ((R + H - 1) / H);
+    size_t b = 0;
+    b += 6 * pool_bytes_of(R * P * es);                         // h, um, vn ping-pong
+    b += pool_bytes_of(R * P * 8);                               // z
+    b += pool_bytes_of(sizeof(Scenario));                        // descriptor
+    b += pool_bytes_of(3 * sizeof(Clock));
+    b += pool_bytes_of(2 * ntiles * sizeof(Partial));
+    b += 2 * pool_bytes_of(sizeof(Ledger));                      // ledger, peek
+    if (hydro_cap > 0) b += pool_bytes_of(2 * (size_t)hydro_cap * 8);
+    if (manning_map) b += pool_bytes_of(R * P);
+    b += 8 * kPoolAlign;                                         // forcing tables, sources, outlet
+    return b;
+}
+
+int sw2d_batch_new(sw2d_batch *b, const sw2d_grid_desc *grid, const double *z, const double *h0,
+                   const sw2d_params *params, sw2d_handle **out)
+{
+    if (!b) return bfail(nullptr, SW2D_E_ARG, "batch is NULL");
+    int rc = create_impl(grid, z, h0, params, b, out);
+    if (rc) b->err = g_create_error;
+    return rc;
+}
+
+int sw2d_batch_attach(sw2d_batch *b, sw2d_handle *h, long max_iters, double t_end)
+{
+    if (!b || !h) return bfail(b, SW2D_E_ARG, "NULL argument");
+    if (h->batch != b) return bfail(b, SW2D_E_ARG, "handle does not belong to this batch");
+    if (h->attached) return bfail(b, SW2D_E_STATE, "already attached");
+    if (b->free_slots.empty()) return bfail(b, SW2D_E_STATE, "batch is full");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    Clock ck;
+    BCHK(b, hipMemcpy(&ck, h->clk + (h->step_index % 3), sizeof(Clock), hipMemcpyDeviceToHost));
+    const int slot = b->free_slots.back();
+    b->free_slots.pop_back();
+    h->member_slot = slot;
+    h->phase0 = (int)(((b->phase - h->step_index) % 6 + 6) % 6);
+    h->k_end = max_iters < 0 ? 0x7fffffffffffffffLL : h->step_index + max_iters;
+    h->t_end = (t_end == t_end) ? t_end : INFINITY;
+    Status s;
+    memset(&s, 0, sizeof s);
+    s.k = h->step_index;
+    s.t = ck.t;
+    BCHK(b, hipMemcpy(b->dstatus + slot, &s, sizeof s, hipMemcpyHostToDevice));
+    h->attached = true;
+    b->members.push_back(h);
+    b->dirty = true;
+    return SW2D_OK;
+}
+
+int sw2d_batch_detach(sw2d_batch *b, sw2d_handle *h)
+{
+    if (!b || !h) return bfail(b, SW2D_E_ARG, "NULL argument");
+    if (h->batch != b || !h->attached) return bfail(b, SW2D_E_STATE, "not attached to this batch");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->members.erase(std::find(b->members.begin(), b->members.end(), h));
+    b->free_slots.push_back(h->member_slot);
+    h->member_slot = -1;
+    h->attached = false;
+    h->desc_dirty = true;
+    b->dirty = true;
+    return SW2D_OK;
+}
+
+int sw2d_batch_step(sw2d_batch *b, int n_iters, int *running)
+{
+    if (!b) return SW2D_E_ARG;
+    if (n_iters < 0) return bfail(b, SW2D_E_ARG, "negative iteration count");
+    int rc = rebuild(b);
+    if (rc) return rc;
+    if (b->members.empty()) {
+        if (running) *running = 0;
+        return SW2D_OK;
+    }
+    long n = n_iters;
+    while (n > 0) {
+        const bool group = b->graph_on && (b->phase % 6 == 0) && n >= 6;
+        if (group && (rc = ensure_bgraph(b))) return rc;
+        const long m = group ? 6 * (n / 6) : 1;
+        const long reps = group ? m / 6 : 1;
+        for (long r = 0; r < reps; r++) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (b->timing) {
+                BCHK(b, hipEventCreate(&e0));
+                BCHK(b, hipEventCreate(&e1));
+                BCHK(b, hipEventRecord(e0, b->stream));
+            }
+            if (group) {
+                BCHK(b, hipGraphLaunch(b->gexec, b->stream));
+            } else {
+                hipError_t e = blaunch(b, b->dscen, b->dtiles, b->nblocks, (int)(b->phase % 6));
+                if (e != hipSuccess) return bhip(b, e, "batch launch");
+            }
+            if (b->timing) {
+                BCHK(b, hipEventRecord(e1, b->stream));
+                b->events.emplace_back(e0, e1);
+                b->launches += group ? 6 : 1;
+                if (b->events.size() >= 1024 && flush_bevents(b)) return SW2D_E_HIP;
+            }
+        }
+        b->phase += m;
+        n -= m;
+    }
+    return refresh(b, running);
+}
+
+int sw2d_batch_status(sw2d_batch *b, const sw2d_handle *h, int *done, long *iters, double *time)
+{
+    if (!b || !h) return bfail(b, SW2D_E_ARG, "NULL argument");
+    if (h->batch != b || !h->attached) return bfail(b, SW2D_E_STATE, "not attached to this batch");
+    const Status &s = b->hstatus[h->member_slot];
+    if (done) *done = s.done || s.fin;
+    if (iters) *iters = (long)s.k;
+    if (time) *time = s.t;
+    return SW2D_OK;
+}
+
+int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
+{
+    if (!b) return SW2D_E_ARG;
+    if (iters < 1) return bfail(b, SW2D_E_ARG, "iters must be positive");
+    if (b->members.empty()) return bfail(b, SW2D_E_STATE, "no members to tune on");
+    for (sw2d_handle *h : b->members)
+        if (h->step_index != 0) return bfail(b, SW2D_E_STATE, "autotune runs before the first iteration");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128};
+    const int nm = (int)b->members.size();
+    // scratch bookkeeping per member, sized for the smallest candidate
+    struct Scratch { Clock *clk; Partial *part; Ledger *led; double *bc; int maxt; };
+    std::vector<Scratch> sc(nm);
+    Scenario *dscen_s = nullptr;
+    TileRef *dtiles_s = nullptr;
+    int rc = SW2D_OK;
+    auto cleanup = [&]() {
+        for (int i = 0; i < nm; i++) {
+            sw2d_handle *h = b->members[i];
+            dfree(h, sc[i].clk); dfree(h, sc[i].part); dfree(h, sc[i].led); dfree(h, sc[i].bc);
+        }
+        if (dscen_s) (void)hipFree(dscen_s);
+        if (dtiles_s) (void)hipFree(dtiles_s);
+    };
+    long maxblocks = 0;
+    for (int i = 0; i < nm; i++) {
+        sw2d_handle *h = b->members[i];
+        sc[i] = Scratch{nullptr, nullptr, nullptr, nullptr, 0};
+        sc[i].maxt = ((h->C + h->W - 1) / h->W) * ((h->R + cand[0] - 1) / cand[0]);
+        maxblocks += sc[i].maxt + 1 + (h->nbc + 63) / 64;
+    }
+    for (int i = 0; i < nm && !rc; i++) {
+        sw2d_handle *h = b->members[i];
+        if ((rc = dalloc(h, &sc[i].clk, 3)) || (rc = dalloc(h, &sc[i].part, 2 * (size_t)sc[i].maxt)) ||
+            (rc = dalloc(h, &sc[i].led, 1)) || (rc = dalloc(h, &sc[i].bc, std::max(h->nbc, 1)))) {
+            b->err = h->err;
+        }
+    }
+    if (rc) { cleanup(); return rc; }
+    hipError_t e = hipMalloc(&dscen_s, sizeof(Scenario) * nm);
+    if (e == hipSuccess) e = hipMalloc(&dtiles_s, sizeof(TileRef) * maxblocks);
+    if (e != hipSuccess) { cleanup(); return bhip(b, e, "hipMalloc"); }
+    hipEvent_t e0, e1;
+    BCHK(b, hipEventCreate(&e0));
+    BCHK(b, hipEventCreate(&e1));
+    int best = b->H;
+    float best_ms = 1e30f;
+    for (int H : cand) {
+        std::vector<Scenario> desc(nm);
+        std::vector<TileRef> tiles;
+        for (int i = 0; i < nm && e == hipSuccess; i++) {
+            sw2d_handle *h = b->members[i];
+            const int H0 = h->H;
+            set_tiles(h, h->prm.tile_rows > 0 ? h->prm.tile_rows : H);
+            Scenario d = make_scenario(h);
+            set_tiles(h, H0);
+            d.s.clk = sc[i].clk;
+            d.part = sc[i].part;
+            d.s.nparts = d.ntiles;
+            d.s.ledger = sc[i].led;
+            d.s.hydro = nullptr;
+            d.s.bc_acc = sc[i].bc;
+            d.phase0 = 0;
+            desc[i] = d;
+            for (int k = 0; k < d.nblocks; k++) tiles.push_back(TileRef{i, k});
+            e = hipMemcpyAsync(sc[i].clk, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, b->stream);
+        }
+        if (e == hipSuccess) e = hipMemcpy(dscen_s, desc.data(), sizeof(Scenario) * nm, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dtiles_s, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice);
+        if (e != hipSuccess) break;
+        // every trial launch is iteration 0 of every member again
+        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0);
+        if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0);
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) break;
+        if (ms < best_ms) { best_ms = ms; best = H; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    cleanup();
+    if (e != hipSuccess) return bhip(b, e, "batch autotune");
+    b->H = best;
+    for (sw2d_handle *h : b->members) {
+        if (h->prm.tile_rows > 0) continue;
+        set_tiles(h, best);
+        if (h->ntiles > h->nparts) {
+            dfree(h, h->part);
+            h->part = nullptr;
+            if ((rc = dalloc(h, &h->part, 2 * (size_t)h->ntiles))) { b->err = h->err; return rc; }
+        }
+        h->nparts = h->ntiles;
+        changed(h);
+    }
+    b->dirty = true;
+    BCHK(b, hipStreamSynchronize(b->stream));
+    if (tile_rows) *tile_rows = best;
+    return SW2D_OK;
+}
+
+int sw2d_batch_memory(sw2d_batch *b, size_t *in_use, size_t *peak, size_t *capacity)
+{
+    if (!b) return SW2D_E_ARG;
+    if (in_use) *in_use = b->pool.in_use();
+    if (peak) *peak = b->pool.peak();
+    if (capacity) *capacity = b->pool.capacity();
+    return SW2D_OK;
+}
+
+int sw2d_batch_timing(sw2d_batch *b, int enable)
+{
+    if (!b) return SW2D_E_ARG;
+    if (flush_bevents(b)) return SW2D_E_HIP;
+    b->kernel_ms = 0.0;
+    b->launches = 0;
+    b->timing = enable ? 1 : 0;
+    return SW2D_OK;
+}
+
+int sw2d_batch_kernel_time(sw2d_batch *b, double *ms, long *launches)
+{
+    if (!b) return SW2D_E_ARG;
+    if (flush_bevents(b)) return SW2D_E_HIP;
+    if (ms) *ms = b->kernel_ms;
+    if (launches) *launches = (long)b->launches;
+    return SW2D_OK;
+}
+
+}  // extern "C"

@@ -6,11 +6,15 @@
 
 namespace sw2d {
 
-hipError_t launch_fused_f32(sw2d_handle *h, const StepCommon &sc) { return launch_fused_impl<float>(h, sc); }
-int fused_waves_per_cu_f32(int adaptive)
+hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
+                           int phase)
+{
+    return launch_fused_impl<float>(stream, scen, tiles, nblocks, phase);
+}
+int fused_waves_per_cu_f32()
 {
     int nb = 0;
-    const void *fn = adaptive ? (const void *)fused_step<float, true> : (const void *)fused_step<float, false>;
+    const void *fn = (const void *)fused_step<float>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
     return nb;
 }

@@ -10,11 +10,15 @@ __global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *ou
     ledger_peek(s, out, threadIdx.x);
 }
 
-hipError_t launch_fused_f64(sw2d_handle *h, const StepCommon &sc) { return launch_fused_impl<double>(h, sc); }
-int fused_waves_per_cu_f64(int adaptive)
+hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
+                           int phase)
+{
+    return launch_fused_impl<double>(stream, scen, tiles, nblocks, phase);
+}
+int fused_waves_per_cu_f64()
 {
     int nb = 0;
-    const void *fn = adaptive ? (const void *)fused_step<double, true> : (const void *)fused_step<double, false>;
+    const void *fn = (const void *)fused_step<double>;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
     return nb;
 }

@@ -43,41 +43,94 @@ __device__ __forceinline__ bool west_bit(bool p, int lane)
 
 constexpr int kMaxTileRows = 512;
 
-template <typename T, bool ADAPT>
-__global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
+template <typename T>
+__global__ void __launch_bounds__(64)
+fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles, const int nblocks,
+           const int phase)
 {
     using M = num<T>;
-    const StepCommon &s = a.s;
     const int lane = threadIdx.x;
-    const StepTime st = step_time(s);
 
-    if ((int)blockIdx.x >= a.ntiles) {
-        PitchedState<T> ps{a.h_in, a.um_in, a.vn_in, s.R, s.C, a.P, (T)s.th};
+    // block -> (scenario, block within it).  The hardware deals consecutive
+    // blocks to the 8 XCDs in turn; the remap gives each XCD one contiguous
+    // run of the block list, so neighbouring tiles share an L2.
+    int b = blockIdx.x;
+    {
+        const int nx = 8, per = nblocks / nx, rem = nblocks % nx;
+        const int x = b % nx, i = b / nx;
+        b = x * per + (x < rem ? x : rem) + i;
+    }
+    int sid = 0, tile = b;
+    if (tiles) {
+        const TileRef t = tiles[b];
+        sid = t.sid;
+        tile = t.local;
+    }
+    const Scenario &d = scen[sid];
+    StepCommon s = d.s;
+    // iteration k of this scenario: k mod 6 from the launch phase, k itself
+    // from the clock slot the previous iteration wrote
+    const int k6 = (phase - d.phase0 + 6) % 6;
+    s.slot_a = k6 % 3; s.slot_b = (k6 + 1) % 3; s.slot_c = (k6 + 2) % 3;
+    s.part_out = d.part + (size_t)(k6 & 1) * s.nparts;
+    s.part_in = d.part + (size_t)((k6 + 1) & 1) * s.nparts;
+    s.step_index = s.clk[s.slot_a].mstep >> 1;
+    const StepTime st = step_time(s);
+    if (d.status) {
+        // batch member: a finished scenario leaves its blocks idle
+        if (d.status->done) return;
+        if (!(s.step_index < d.k_end && st.t + st.dt <= d.t_end)) {
+            if (tile == d.ntiles && lane == 0) d.status->done = 1;
+            return;
+        }
+    }
+    const int ib = k6 & 1;
+    // the descriptor's pointers are global memory: saying so lets the
+    // compiler address them as SGPR base + VGPR offset (global_load) instead
+    // of 64-bit flat addresses in VGPRs
+    const gptr<const T> h_in = (gptr<const T>)d.st[ib][0];
+    const gptr<const T> um_in = (gptr<const T>)d.st[ib][1];
+    const gptr<const T> vn_in = (gptr<const T>)d.st[ib][2];
+    const gptr<T> h_out = (gptr<T>)d.st[ib ^ 1][0];
+    const gptr<T> um_out = (gptr<T>)d.st[ib ^ 1][1];
+    const gptr<T> vn_out = (gptr<T>)d.st[ib ^ 1][2];
+    const gptr<const double> zg = (gptr<const double>)d.z;
+    const gptr<const uint8_t> clsg = (gptr<const uint8_t>)d.cls;
+
+    if (tile >= d.ntiles) {
+        PitchedState<T> ps{(const T *)d.st[ib][0], (const T *)d.st[ib][1], (const T *)d.st[ib][2],
+                           s.R, s.C, d.P, (T)s.th};
         auto vel = [&](int r, int j, double &hh, double &ua, double &va) {
             hh = (double)ps.hv(r, j);
             ua = (double)ps.uua(r, j);
             va = (double)ps.vva(r, j);
         };
-        bookkeeping(s, st, vel, (int)blockIdx.x - a.ntiles, lane);
+        const int part = tile - d.ntiles;
+        bookkeeping(s, st, vel, part, lane);
+        if (part == 0 && lane == 0 && d.status) {
+            const long long kn = s.step_index + 1;
+            const double tn = st.tm + st.dt;
+            d.status->k = kn;
+            d.status->t = tn;
+            // the flag the kernel reads (done) is only set by a later launch;
+            // fin tells the host now that nothing more will run
+            if (kn >= d.k_end || (s.fixed_dt && !(tn + s.dt_fixed <= d.t_end))) d.status->fin = 1;
+        }
         return;
     }
 
-    // tile of this wave; with xcd_remap consecutive tiles go to one XCD
-    int tile = blockIdx.x;
-    if (a.xcd_remap) {
-        const int nx = 8, per = a.ntiles / nx, rem = a.ntiles % nx;
-        const int x = tile % nx, i = tile / nx;
-        tile = x * per + (x < rem ? x : rem) + i;
-    }
-    const int strip = tile % a.nstrips, chunk = tile / a.nstrips;
-    const int R = s.R, C = s.C, P = a.P;
-    const int j0 = strip * a.W;
+    const int strip = tile % d.nstrips, chunk = tile / d.nstrips;
+    const int R = s.R, C = s.C, P = d.P;
+    const int j0 = strip * d.W;
     const int j = j0 - 3 + lane;
     const bool colin = (j >= 0) && (j < C);
-    const bool colout = (lane >= 3) && (lane < 3 + a.W) && (j < C);
-    const int r0 = chunk * a.H;
-    const int r1 = (r0 + a.H < R) ? r0 + a.H : R;
+    const bool colout = (lane >= 3) && (lane < 3 + d.W) && (j < C);
+    const int r0 = chunk * d.H;
+    const int r1 = (r0 + d.H < R) ? r0 + d.H : R;
 
+    // CFL candidates only for an adaptive time step; a wave-uniform flag, so
+    // a scenario runs the same instructions alone and in a mixed batch
+    const bool adapt = !s.fixed_dt;
     const T th = (T)s.th, gg = (T)s.g, dx = (T)s.dx, dy = (T)s.dy;
     const T rdx = (T)(1.0 / s.dx), rdy = (T)(1.0 / s.dy);
     const T dt2 = (T)st.dt2;
@@ -94,7 +147,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         const double x = ((double)j + 0.5) * s.dx;
         gx1 = storm_factor(x - (s.st_x0 + s.st_vx * st.t), s.st_sigma);
         gx2 = storm_factor(x - (s.st_x0 + s.st_vx * st.tm), s.st_sigma);
-        for (int i = lane; i < a.H + 6; i += 64) {
+        for (int i = lane; i < d.H + 6; i += 64) {
             const double y1 = ((double)(r0 - 5 + i) + 0.5) * s.dy;
             const double y2 = ((double)(r0 - 6 + i) + 0.5) * s.dy;
             gy_s[0][i] = storm_factor(y1 - (s.st_y0 + s.st_vy * st.t), s.st_sigma);
@@ -134,13 +187,13 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
         const int off = kc * P + jc;
-        const T lh = a.h_in[off], lu = a.um_in[off], lv = a.vn_in[off];
-        const double lz = a.z[off];
+        const T lh = h_in[off], lu = um_in[off], lv = vn_in[off];
+        const double lz = zg[off];
         hh = in ? lh : T(0);
         uu = in ? lu : T(0);
         vv = in ? lv : T(0);
         zz = in ? lz : __builtin_nan("");
-        cc = (s.ncls_on && in) ? a.cls[off] : (uint8_t)0;
+        cc = (s.ncls_on && in) ? clsg[off] : (uint8_t)0;
     };
     T nh, nu, nv;
     double nz;
@@ -153,7 +206,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
-        const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
+        const T nc0 = s.ncls_on ? (T)d.s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
         const T h0w = from_west(h0);
@@ -174,7 +227,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         const T uua1 = (j > 0) ? T(0.250) * (uu1_1 + uu1_0 + uu1_1w + uu1_0w)
                                : T(0.50) * (uu1_1 + uu1_0);
 
-        if (ADAPT && colout && k >= r0 && k < r1) {
+        if (adapt && colout && k >= r0 && k < r1) {
             // CFL candidates of the incoming state on the faces of row k
             if (k >= 1 && a0 && a1 && hm0 > th) {
                 const double w = (double)M::abs_(uu1_0) + sqrt(s.g * (double)hm0);
@@ -351,9 +404,9 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
             if (!a3) h2v = 0.00;
             if (colout && q >= r0 && q < r1) {
                 const long off = (long)q * P + j;
-                a.h_out[off] = h2v;
-                a.um_out[off] = umpP;
-                a.vn_out[off] = vnpP;
+                h_out[off] = h2v;
+                um_out[off] = umpP;
+                vn_out[off] = vnpP;
                 if (a3) {
                     acc_vol += (double)h2v;
                     acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
@@ -383,7 +436,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
     acc_clip = wave_sum(acc_clip);
     acc_q = wave_sum(acc_q);
     double cfl = INFINITY;
-    if (ADAPT) {
+    if (adapt) {
         smax_x = wave_max(smax_x);
         smax_y = wave_max(smax_y);
         const double wx = smax_x > 0.0 ? s.dx / smax_x : INFINITY;
@@ -394,7 +447,7 @@ __global__ void __launch_bounds__(64) fused_step(const FusedArgs<T> a)
         Partial p;
         p.vol = acc_vol; p.src = acc_src; p.clip = acc_clip; p.qout = acc_q;
         s.part_out[tile] = p;
-        if (ADAPT && cfl < INFINITY)
+        if (adapt && cfl < INFINITY)
             atomicMin(&s.clk[s.slot_b].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
     }
 }
@@ -470,24 +523,10 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
 }
 
 template <typename T>
-static hipError_t launch_fused_impl(sw2d_handle *h, const StepCommon &sc)
+static hipError_t launch_fused_impl(hipStream_t stream, const Scenario *scen, const TileRef *tiles,
+                                    int nblocks, int phase)
 {
-    FusedArgs<T> a;
-    const int in = h->cur, outb = 1 - h->cur;
-    a.h_in = (const T *)h->h[in]; a.um_in = (const T *)h->um[in]; a.vn_in = (const T *)h->vn[in];
-    a.h_out = (T *)h->h[outb]; a.um_out = (T *)h->um[outb]; a.vn_out = (T *)h->vn[outb];
-    a.z = (const double *)h->z;
-    a.cls = h->cls;
-    a.P = h->P; a.W = h->W; a.H = h->H;
-    a.nstrips = h->nstrips; a.ntiles = h->ntiles;
-    a.xcd_remap = 1;
-    a.s = sc;
-    // one wave per tile, one for clock + ledger, one per 64 lake-inflow records
-    dim3 grid(h->ntiles + 1 + (h->nbc + 63) / 64), block(64);
-    if (h->prm.adaptive)
-        hipLaunchKernelGGL((fused_step<T, true>), grid, block, 0, h->stream, a);
-    else
-        hipLaunchKernelGGL((fused_step<T, false>), grid, block, 0, h->stream, a);
+    hipLaunchKernelGGL(fused_step<T>, dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
     return hipGetLastError();
 }
 

@@ -19,8 +19,13 @@
 #include <vector>
 
 #include "../../include/sw2d.h"
+#include "sw2d_pool.hpp"
 
 namespace sw2d {
+
+// pointer into global memory (address space 1 of the AMDGPU target)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
 
 constexpr unsigned long long kInfBits = 0x7ff0000000000000ULL;
 constexpr int kMaxClasses = 8;
@@ -77,21 +82,46 @@ struct StepCommon {
     int nbc; const int *bc_r; const int *bc_c; const int *bc_kind; double *bc_acc;
 };
 
-template <typename T>
-struct FusedArgs {
-    const T *h_in; const T *um_in; const T *vn_in;
-    T *h_out; T *um_out; T *vn_out;
+// Progress of a scenario stepped inside a batch, written by its bookkeeping
+// wave after every iteration it runs.
+struct Status {
+    long long k;     // iterations completed
+    double t;        // time after them
+    int done;        // set by the first launch whose iteration would pass k_end / t_end
+    int fin;         // set with the last iteration when that is already known (the
+                     // iteration bound, or the time bound with a fixed step)
+};
+
+// Device descriptor of one scenario for the fused kernel.  Everything that
+// changes from one iteration to the next is derived inside the kernel from the
+// launch phase (iteration k mod 6, which fixes the clock slots k%3.. and the
+// ping-pong buffer k%2) and from the clock ring itself (k = mstep / 2), so a
+// fixed sequence of six launches can be captured once and replayed.
+struct Scenario {
+    StepCommon s;            // slot_*, step_index and part_in/out are filled per launch
+    void *st[2][3];          // (h, um, vn) of ping-pong buffer 0 and 1; iteration k reads k%2
     const double *z;         // bed elevation (FP64 in both forms); NaN marks an inactive cell
     const uint8_t *cls;      // Manning class per cell (ncls_on)
     int P;                   // row pitch in elements
     int W;                   // output columns per tile (64 - 6)
     int H;                   // output rows per tile
-    int nstrips, ntiles;
-    int xcd_remap;
-    StepCommon s;
+    int nstrips, ntiles;     // tiles of this scenario
+    int nblocks;             // tiles + bookkeeping waves
+    Partial *part;           // two halves of nparts records
+    int phase0;              // launch phase of this scenario's iteration 0 (mod 6)
+    long long k_end;         // a batch member stops before iteration k_end ...
+    double t_end;            // ... or before an iteration that would end past t_end + dt
+    Status *status;          // batch members only; NULL for a stand-alone handle
+};
+
+// one block of a batched launch: scenario and block within it
+struct TileRef {
+    int sid, local;
 };
 
 }  // namespace sw2d
+
+struct sw2d_batch;
 
 // the opaque handle of the C ABI
 struct sw2d_handle {
@@ -137,6 +167,20 @@ struct sw2d_handle {
     long long step_index = 0;
     double host_time = 0.0;
     long long host_mstep = 0;
+    // fused-path launch descriptor (device copy of make_scenario) and the
+    // six-launch graph that replays iterations k..k+5 for k % 6 == 0
+    sw2d::Scenario *dscen = nullptr;
+    bool desc_dirty = true;
+    bool graph_on = true;
+    hipGraphExec_t gexec = nullptr;
+    // batch membership (sw2d_batch_new): memory from the batch pool, the
+    // batch's stream, stepped by sw2d_batch_step while attached
+    sw2d_batch *batch = nullptr;
+    bool attached = false;
+    int member_slot = -1;
+    int phase0 = 0;                     // batch launch phase of this member's iteration 0 (mod 6)
+    long long k_end = 0;                // stop before this iteration ...
+    double t_end = 0.0;                 // ... or before one that would end past t_end + dt
     // timing
     int timing = 0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
@@ -146,17 +190,51 @@ struct sw2d_handle {
     std::string err;
 };
 
+// a batch: scenarios stepped together, one launch per iteration for all
+struct sw2d_batch {
+    int device = 0;
+    int precision = SW2D_FP32;
+    int esize = 4;
+    hipStream_t stream = nullptr;
+    char *pool_base = nullptr;
+    sw2d::Pool pool;
+    int H = 0;                              // member tile rows (0: each member's default)
+    std::vector<sw2d_handle *> members;     // attached, in attach order
+    std::vector<sw2d_handle *> owned;       // every handle made by sw2d_batch_new and not destroyed
+    std::vector<int> free_slots;            // status / descriptor slots
+    sw2d::Scenario *dscen = nullptr;        // kMaxMembers descriptors
+    sw2d::Status *dstatus = nullptr;        // kMaxMembers status records
+    sw2d::Status *hstatus = nullptr;        // pinned host copy
+    sw2d::TileRef *dtiles = nullptr;
+    int tiles_cap = 0;
+    int nblocks = 0;
+    bool dirty = true;                      // descriptor / tile tables need a rebuild
+    long long phase = 0;                    // launches issued (mod 6 is the launch phase)
+    hipGraphExec_t gexec = nullptr;
+    bool graph_on = true;
+    int timing = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+    long long launches = 0;
+    double kernel_ms = 0.0;
+    std::string err;
+};
+
 namespace sw2d {
+constexpr int kMaxMembers = 4096;
 // launchers (sw2d_fused64.hip, sw2d_fused32.hip, sw2d_plain.hip)
-hipError_t launch_fused_f64(sw2d_handle *h, const StepCommon &sc);
-hipError_t launch_fused_f32(sw2d_handle *h, const StepCommon &sc);
+// one iteration of every scenario in scen[] (tiles == NULL: the single
+// scenario scen[0], blocks in order) at launch phase `phase`
+hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
+                           int phase);
+hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
+                           int phase);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
 hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev);
 hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev);
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc);
-int fused_waves_per_cu_f64(int adaptive);  // resident one-wave blocks per CU
-int fused_waves_per_cu_f32(int adaptive);
+int fused_waves_per_cu_f64();  // resident one-wave blocks per CU
+int fused_waves_per_cu_f32();
 hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc);
 hipError_t launch_plain_init(sw2d_handle *h);
 }  // namespace sw2d

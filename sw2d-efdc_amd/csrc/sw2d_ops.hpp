@@ -1,0 +1,94 @@
+// sw2d_ops.hpp -- host-side helpers shared by the handle (sw2d_api.hip) and
+// batch (sw2d_batch.hip) implementations of the C ABI.
+#pragma once
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "sw2d_internal.hpp"
+
+namespace sw2d {
+
+inline thread_local std::string g_create_error;
+
+inline int fail(sw2d_handle *h, int code, const std::string &msg)
+{
+    if (h) h->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+inline int hip_fail(sw2d_handle *h, hipError_t e, const char *what)
+{
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return fail(h, SW2D_E_HIP, m);
+}
+
+#define HIPCHK(h, expr)                                   \
+    do {                                                  \
+        hipError_t e_ = (expr);                           \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+
+// device arrays of a handle: hipMalloc, or the pool of its batch; zeroed
+// in stream order (a pool range freed by a retired scenario is reused only
+// after the launches queued before it)
+template <typename T>
+inline int dalloc(sw2d_handle *h, T **p, size_t count)
+{
+    void *q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e;
+    if (h->batch) {
+        q = h->batch->pool.alloc(bytes);
+        if (!q) return fail(h, SW2D_E_NOMEM, "batch memory pool exhausted");
+    } else {
+        e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(h, SW2D_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    e = hipMemsetAsync(q, 0, bytes, h->stream);
+    if (e != hipSuccess) return hip_fail(h, e, "hipMemsetAsync");
+    h->allocs.push_back(q);
+    *p = (T *)q;
+    return SW2D_OK;
+}
+
+inline void dfree(sw2d_handle *h, void *p)
+{
+    if (!p) return;
+    auto it = std::find(h->allocs.begin(), h->allocs.end(), p);
+    if (it != h->allocs.end()) h->allocs.erase(it);
+    if (h->batch) (void)h->batch->pool.release(p);
+    else (void)hipFree(p);
+}
+
+inline void drop_graph(sw2d_handle *h)
+{
+    if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+    h->gexec = nullptr;
+}
+
+// a setter changed what the kernels read: refresh the descriptor before the
+// next launch and capture the graph again (its grid size may have changed)
+inline void changed(sw2d_handle *h)
+{
+    h->desc_dirty = true;
+    drop_graph(h);
+    if (h->batch) h->batch->dirty = true;
+}
+
+// defined in sw2d_api.hip
+int round_up(int x, int m);
+StepCommon make_common(sw2d_handle *h);
+bool fused(const sw2d_handle *h);
+int fused_blocks(const sw2d_handle *h);
+Scenario make_scenario(sw2d_handle *h);
+int choose_tile_rows(const sw2d_handle *h);
+void set_tiles(sw2d_handle *h, int H);
+void advance(sw2d_handle *h, long m);
+int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
+                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out);
+
+}  // namespace sw2d

@@ -46,6 +46,33 @@ class Balance(ct.Structure):
                                            "v_outflow", "q_out", "q_peak", "t_peak", "error")]
 
 
+class Request(ct.Structure):
+    _fields_ = [("grid", GridDesc), ("z", ct.POINTER(ct.c_double)), ("h0", ct.POINTER(ct.c_double)),
+                ("params", Params), ("rain_mm", ct.POINTER(ct.c_double)), ("nrain", ct.c_int),
+                ("dtrain", ct.c_double), ("outlet_row", ct.c_int), ("outlet_col", ct.c_int),
+                ("manning_class", ct.POINTER(ct.c_ubyte)),
+                ("manning_table", ct.POINTER(ct.c_double)), ("nclass", ct.c_int),
+                ("t_end", ct.c_double), ("max_iters", ct.c_long), ("hydro_cap", ct.c_long)]
+
+
+class Result(ct.Structure):
+    _fields_ = [("state", ct.c_int), ("t_submit", ct.c_double), ("t_admit", ct.c_double),
+                ("t_done", ct.c_double), ("iterations", ct.c_long), ("time", ct.c_double),
+                ("hydro_n", ct.c_long), ("balance", Balance)]
+
+
+class ServerStats(ct.Structure):
+    _fields_ = [("submitted", ct.c_long), ("completed", ct.c_long), ("failed", ct.c_long),
+                ("running", ct.c_long), ("queued", ct.c_long), ("cell_updates", ct.c_double),
+                ("busy_seconds", ct.c_double), ("uptime", ct.c_double),
+                ("p50_latency", ct.c_double), ("p99_latency", ct.c_double),
+                ("max_latency", ct.c_double), ("p50_wait", ct.c_double), ("p99_wait", ct.c_double),
+                ("pool_peak", ct.c_size_t), ("pool_capacity", ct.c_size_t),
+                ("launches", ct.c_long), ("kernel_ms", ct.c_double)]
+
+
+REQ_STATE = {0: "queued", 1: "running", 2: "done", -1: "failed"}
+
 _D = ct.c_double
 _DP = ct.POINTER(ct.c_double)
 _IP = ct.POINTER(ct.c_int)
@@ -100,6 +127,32 @@ def lib():
         "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
         "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
         "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
+        "sw2d_batch_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.POINTER(_H)]),
+        "sw2d_batch_destroy": (ct.c_int, [_H]),
+        "sw2d_batch_last_error": (ct.c_char_p, [_H]),
+        "sw2d_batch_footprint": (ct.c_size_t, [ct.POINTER(GridDesc), ct.POINTER(Params), ct.c_int,
+                                               ct.c_long, ct.c_int]),
+        "sw2d_batch_new": (ct.c_int, [_H, ct.POINTER(GridDesc), _DP, _DP, ct.POINTER(Params),
+                                      ct.POINTER(_H)]),
+        "sw2d_batch_attach": (ct.c_int, [_H, _H, ct.c_long, _D]),
+        "sw2d_batch_detach": (ct.c_int, [_H, _H]),
+        "sw2d_batch_step": (ct.c_int, [_H, ct.c_int, _IP]),
+        "sw2d_batch_status": (ct.c_int, [_H, _H, _IP, ct.POINTER(ct.c_long), _DP]),
+        "sw2d_batch_autotune": (ct.c_int, [_H, ct.c_int, _IP]),
+        "sw2d_batch_memory": (ct.c_int, [_H, ct.POINTER(ct.c_size_t), ct.POINTER(ct.c_size_t),
+                                         ct.POINTER(ct.c_size_t)]),
+        "sw2d_batch_timing": (ct.c_int, [_H, ct.c_int]),
+        "sw2d_batch_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
+        "sw2d_server_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.c_int,
+                                          ct.POINTER(_H)]),
+        "sw2d_server_submit": (ct.c_int, [_H, ct.POINTER(Request), ct.POINTER(ct.c_long)]),
+        "sw2d_server_poll": (ct.c_int, [_H, ct.c_long, ct.POINTER(Result)]),
+        "sw2d_server_hydrograph": (ct.c_int, [_H, ct.c_long, _DP, ct.c_long, ct.POINTER(ct.c_long)]),
+        "sw2d_server_release": (ct.c_int, [_H, ct.c_long]),
+        "sw2d_server_wait": (ct.c_int, [_H, ct.c_long, _D]),
+        "sw2d_server_stats_get": (ct.c_int, [_H, ct.POINTER(ServerStats)]),
+        "sw2d_server_last_error": (ct.c_char_p, [_H]),
+        "sw2d_server_destroy": (ct.c_int, [_H]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -175,39 +228,64 @@ def evaporation_series(albedo, T, Rg, dtime):
     return out
 
 
+def make_params(*, dt, manning, g=9.80, th=1.0e-4, INT=0.0, INF=0.0, LWL=0.0, EV_WL_min=0.0,
+                time0=0.0, precision="fp64", kernel="fused", adaptive=False, courant=0.5,
+                device=0, tile_rows=0):
+    p = Params()
+    lib().sw2d_default_params(ct.byref(p))
+    p.g, p.manning, p.th, p.dt, p.time0 = g, manning, th, dt, time0
+    p.INT, p.INF, p.LWL, p.EV_WL_min = INT, INF, LWL, EV_WL_min
+    p.precision = {"fp64": FP64, "fp32": FP32}[precision]
+    p.kernel = {"fused": FUSED, "plain": PLAIN}[kernel]
+    p.adaptive = 1 if adaptive else 0
+    p.courant = courant
+    p.device = device
+    p.tile_rows = tile_rows
+    return p
+
+
+def footprint(rows, cols, *, precision="fp32", tile_rows=0, hydro_cap=0, manning_map=False,
+              batch_tile_rows=0):
+    """Pool bytes a batch member of this shape takes (sw2d_batch_footprint)."""
+    gd = GridDesc(rows, cols, 0.0, 0.0, 1.0, -9999.0)
+    p = make_params(dt=0.5, manning=0.03, precision=precision, tile_rows=tile_rows)
+    return int(lib().sw2d_batch_footprint(ct.byref(gd), ct.byref(p), batch_tile_rows, hydro_cap,
+                                          1 if manning_map else 0))
+
+
 class Case:
-    """One SW2D case resident on the GPU (a sw2d_handle)."""
+    """One SW2D case resident on the GPU (a sw2d_handle).  With batch=Batch(...)
+    its arrays live in the batch's memory pool (sw2d_batch_new)."""
 
     def __init__(self, z, h0, cellsize, *, dt, manning, nodata=-9999.0, xll=0.0, yll=0.0,
-                 g=9.80, th=1.0e-4, INT=0.0, INF=0.0, LWL=0.0, EV_WL_min=0.0, time0=0.0,
-                 precision="fp64", kernel="fused", adaptive=False, courant=0.5, device=0,
-                 tile_rows=0):
+                 batch=None, **kw):
         self._L = lib()
         z = _f64(z)
         h0 = _f64(h0)
         self.R, self.C = z.shape
         gd = GridDesc(self.R, self.C, xll, yll, cellsize, nodata)
-        p = Params()
-        self._L.sw2d_default_params(ct.byref(p))
-        p.g, p.manning, p.th, p.dt, p.time0 = g, manning, th, dt, time0
-        p.INT, p.INF, p.LWL, p.EV_WL_min = INT, INF, LWL, EV_WL_min
-        p.precision = {"fp64": FP64, "fp32": FP32}[precision]
-        p.kernel = {"fused": FUSED, "plain": PLAIN}[kernel]
-        p.adaptive = 1 if adaptive else 0
-        p.courant = courant
-        p.device = device
-        p.tile_rows = tile_rows
+        p = make_params(dt=dt, manning=manning, **kw)
         self._h = _H()
-        rc = self._L.sw2d_create(ct.byref(gd), _dp(z), _dp(h0), ct.byref(p), ct.byref(self._h))
+        self.batch = batch
+        if batch is None:
+            rc = self._L.sw2d_create(ct.byref(gd), _dp(z), _dp(h0), ct.byref(p), ct.byref(self._h))
+            err = self._L.sw2d_last_error(None)
+        else:
+            rc = self._L.sw2d_batch_new(batch._b, ct.byref(gd), _dp(z), _dp(h0), ct.byref(p),
+                                        ct.byref(self._h))
+            err = self._L.sw2d_batch_last_error(batch._b)
         if rc:
-            raise Sw2dError(f"sw2d_create failed ({rc}): "
-                            f"{self._L.sw2d_last_error(None).decode()}")
+            raise Sw2dError(f"sw2d_create failed ({rc}): {err.decode()}")
+        if batch is not None:
+            batch._cases.append(self)
         self.nbc = 0
 
     def close(self):
         if getattr(self, "_h", None):
             self._L.sw2d_destroy(self._h)
             self._h = None
+            if self.batch is not None and self in self.batch._cases:
+                self.batch._cases.remove(self)
 
     def __del__(self):
         self.close()
@@ -319,3 +397,158 @@ class Case:
         b = ct.c_longlong(0)
         self._chk(self._L.sw2d_device_bytes(self._h, ct.byref(b)), "device_bytes")
         return b.value
+
+
+class Batch:
+    """Scenarios stepped together by one launch per iteration (sw2d_batch_*)."""
+
+    def __init__(self, precision="fp32", pool_bytes=1 << 30, tile_rows=0, device=0):
+        self._L = lib()
+        self._b = _H()
+        self._cases = []
+        rc = self._L.sw2d_batch_create(device, {"fp64": FP64, "fp32": FP32}[precision],
+                                       pool_bytes, tile_rows, ct.byref(self._b))
+        if rc:
+            raise Sw2dError(f"sw2d_batch_create failed ({rc}): "
+                            f"{self._L.sw2d_batch_last_error(None).decode()}")
+        self.precision = precision
+
+    def close(self):
+        if getattr(self, "_b", None):
+            for c in list(self._cases):
+                c.close()
+            self._L.sw2d_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise Sw2dError(f"{what} failed ({rc}): {self._L.sw2d_batch_last_error(self._b).decode()}")
+
+    def case(self, z, h0, cellsize, **kw):
+        kw.setdefault("precision", self.precision)
+        return Case(z, h0, cellsize, batch=self, **kw)
+
+    def attach(self, case, max_iters=-1, t_end=float("inf")):
+        self._chk(self._L.sw2d_batch_attach(self._b, case._h, max_iters, t_end), "attach")
+
+    def detach(self, case):
+        self._chk(self._L.sw2d_batch_detach(self._b, case._h), "detach")
+
+    def step(self, n=1):
+        running = ct.c_int(0)
+        self._chk(self._L.sw2d_batch_step(self._b, n, ct.byref(running)), "batch step")
+        return running.value
+
+    def status(self, case):
+        d = ct.c_int(0)
+        k = ct.c_long(0)
+        t = ct.c_double(0)
+        self._chk(self._L.sw2d_batch_status(self._b, case._h, ct.byref(d), ct.byref(k),
+                                            ct.byref(t)), "status")
+        return bool(d.value), k.value, t.value
+
+    def autotune(self, iters=10):
+        best = ct.c_int(0)
+        self._chk(self._L.sw2d_batch_autotune(self._b, iters, ct.byref(best)), "batch autotune")
+        return best.value
+
+    def memory(self):
+        a, b, c = ct.c_size_t(0), ct.c_size_t(0), ct.c_size_t(0)
+        self._chk(self._L.sw2d_batch_memory(self._b, ct.byref(a), ct.byref(b), ct.byref(c)),
+                  "memory")
+        return a.value, b.value, c.value
+
+    def timing(self, enable=True):
+        self._chk(self._L.sw2d_batch_timing(self._b, 1 if enable else 0), "timing")
+
+    def kernel_time(self):
+        ms = ct.c_double(0)
+        n = ct.c_long(0)
+        self._chk(self._L.sw2d_batch_kernel_time(self._b, ct.byref(ms), ct.byref(n)),
+                  "kernel_time")
+        return ms.value, n.value
+
+
+class Server:
+    """Continuous-batching request loop on a device thread (sw2d_server_*)."""
+
+    def __init__(self, precision="fp32", pool_bytes=8 << 30, tile_rows=32, slice_iters=60,
+                 device=0):
+        self._L = lib()
+        self._s = _H()
+        rc = self._L.sw2d_server_create(device, {"fp64": FP64, "fp32": FP32}[precision],
+                                        pool_bytes, tile_rows, slice_iters, ct.byref(self._s))
+        if rc:
+            raise Sw2dError(f"sw2d_server_create failed ({rc}): "
+                            f"{self._L.sw2d_server_last_error(None).decode()}")
+
+    def close(self):
+        if getattr(self, "_s", None):
+            self._L.sw2d_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise Sw2dError(f"{what} failed ({rc}): {self._L.sw2d_server_last_error(self._s).decode()}")
+
+    def submit(self, z, cellsize, *, t_end, h0=None, rain=None, dtrain=1.0, outlet=None,
+               manning_map=None, max_iters=-1, hydro_cap=0, nodata=-9999.0, **kw):
+        z = _f64(z)
+        R, C = z.shape
+        q = Request()
+        q.grid = GridDesc(R, C, 0.0, 0.0, cellsize, nodata)
+        q.z = _dp(z)
+        h0a = _f64(h0) if h0 is not None else None
+        q.h0 = _dp(h0a) if h0a is not None else None
+        q.params = make_params(**kw)
+        ra = _f64(rain) if rain is not None else np.zeros(0)
+        q.rain_mm = _dp(ra) if len(ra) else None
+        q.nrain = len(ra)
+        q.dtrain = dtrain
+        q.outlet_row, q.outlet_col = outlet if outlet is not None else (-1, -1)
+        if manning_map is not None:
+            cls, tab = manning_map
+            ca = np.ascontiguousarray(cls, dtype=np.uint8)
+            ta = _f64(tab)
+            q.manning_class = ca.ctypes.data_as(ct.POINTER(ct.c_ubyte))
+            q.manning_table = _dp(ta)
+            q.nclass = len(ta)
+        q.t_end = t_end
+        q.max_iters = max_iters
+        q.hydro_cap = hydro_cap
+        rid = ct.c_long(0)
+        self._chk(self._L.sw2d_server_submit(self._s, ct.byref(q), ct.byref(rid)), "submit")
+        return rid.value
+
+    def poll(self, rid):
+        r = Result()
+        self._chk(self._L.sw2d_server_poll(self._s, rid, ct.byref(r)), "poll")
+        out = {n: getattr(r, n) for n, _ in Result._fields_ if n != "balance"}
+        out["state"] = REQ_STATE[r.state]
+        out["balance"] = {n: getattr(r.balance, n) for n, _ in Balance._fields_}
+        return out
+
+    def hydrograph(self, rid):
+        n = ct.c_long(0)
+        self._chk(self._L.sw2d_server_hydrograph(self._s, rid, None, 0, ct.byref(n)), "hydrograph")
+        out = np.zeros(2 * max(n.value, 1))
+        self._chk(self._L.sw2d_server_hydrograph(self._s, rid, _dp(out), n.value, ct.byref(n)),
+                  "hydrograph")
+        return out[: 2 * n.value].reshape(n.value, 2)
+
+    def release(self, rid):
+        self._chk(self._L.sw2d_server_release(self._s, rid), "release")
+
+    def wait(self, rid=-1, timeout=600.0):
+        self._chk(self._L.sw2d_server_wait(self._s, rid, timeout), "wait")
+
+    def stats(self):
+        st = ServerStats()
+        self._chk(self._L.sw2d_server_stats_get(self._s, ct.byref(st)), "stats")
+        return {n: getattr(st, n) for n, _ in ServerStats._fields_}

@@ -1,0 +1,184 @@
+"""Batches: scenarios of different sizes stepped by one launch per iteration.
+
+The bar is identity with the same scenario stepped alone: every member keeps
+the single-grid semantics, so its state (h, um, vn) after a batch run is
+equal value for value to that of a stand-alone handle run for the same
+iterations (same tile rows, so the ledger's summation order is the same too),
+whatever else shares the launches and whenever it joined.  FP64 members are
+also compared with the CPU FP64 oracle directly.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("h", "um", "vn")
+
+
+def dem(cases, R, C, seed, dx=10.0, slope=0.01, amp=2.0, holes=False):
+    r = np.arange(R, dtype=np.float64)[:, None]
+    z = slope * dx * (R - 1 - r) + cases.fbm(R, C, 0.7, amp, seed)
+    z = np.broadcast_to(z, (R, C)).copy()
+    if holes:
+        rng = np.random.default_rng(seed + 3)
+        z[rng.random((R, C)) < 0.02] = -9999.0
+    return z
+
+
+def scenario(cases, i):
+    """A small mixed set: sizes, rain, outlet, Manning map, adaptive step."""
+    shapes = [(40, 70), (96, 50), (130, 131), (23, 200), (64, 64)]
+    R, C = shapes[i % len(shapes)]
+    z = dem(cases, R, C, 100 + i, holes=(i == 2))
+    h0 = np.zeros((R, C))
+    if i == 4:
+        h0[R // 4:R // 2, C // 4:C // 2] = 0.5
+    kw = dict(adaptive=(i % 2 == 1), dt=0.5, manning=0.03 + 0.01 * i)
+    rain = np.array([30.0 + 10 * i, 5.0, 0.0, 0.0])
+    outlet = (R - 1, int(np.argmin(np.where(z[R - 1] > -9000, z[R - 1], np.inf))))
+    return z, h0, kw, rain, outlet
+
+
+def setup(case, cases, i, rain, outlet):
+    case.set_rain(rain, 60.0)
+    case.set_outlet(*outlet)
+    case.set_hydrograph(4096)
+    if i == 3:
+        cls = (np.arange(case.R * case.C).reshape(case.R, case.C) % 3).astype(np.uint8)
+        case.set_manning_map(cls, [0.02, 0.05, 0.08])
+
+
+def assert_same(a, b, tag):
+    for f in STATE:
+        x, y = a.field(f), b.field(f)
+        if not np.array_equal(x, y):
+            d = np.argwhere(x != y)
+            raise AssertionError(f"{tag} {f}: {len(d)} cells differ, first {tuple(d[0])} "
+                                 f"{x[tuple(d[0])]!r} vs {y[tuple(d[0])]!r}")
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_batch_equals_alone(pkg, cases, precision):
+    H = 16
+    iters = [37, 60, 25, 71, 48]
+    join = [0, 0, 5, 13, 0]   # batch iterations before a member is attached (phases 5, 1)
+    batch = pkg.Batch(precision=precision, pool_bytes=256 << 20, tile_rows=H)
+    alone, members = [], []
+    for i in range(5):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision=precision, tile_rows=H, **kw)
+        setup(a, cases, i, rain, outlet)
+        a.step(iters[i])
+        alone.append(a)
+        m = batch.case(z, h0, 10.0, **kw)
+        setup(m, cases, i, rain, outlet)
+        members.append(m)
+    attached = [False] * 5
+    k, chunk = 0, 0
+    while True:
+        for i in range(5):
+            if not attached[i] and join[i] <= k:
+                batch.attach(members[i], max_iters=iters[i])
+                attached[i] = True
+        n = (5, 8)[chunk % 2]   # ragged chunks: graph groups and single launches
+        running = batch.step(n)
+        k, chunk = k + n, chunk + 1
+        for i in range(5):
+            if attached[i]:
+                assert batch.status(members[i])[1] <= iters[i]
+        if all(attached) and running == 0:
+            break
+        assert k < 400
+    for i in range(5):
+        assert batch.status(members[i])[1] == iters[i]
+        assert_same(members[i], alone[i], f"{precision} member {i}")
+        assert members[i].time == alone[i].time
+        ha, hm = alone[i].hydrograph(), members[i].hydrograph()
+        assert np.array_equal(ha, hm), f"member {i} hydrograph"
+        ba, bm = alone[i].balance(), members[i].balance()
+        assert ba == bm, f"member {i} ledger"
+    batch.close()
+
+
+def test_batch_fp64_matches_oracle(pkg, po, cases):
+    """FP64 members against the CPU FP64 oracle, value for value."""
+    batch = pkg.Batch(precision="fp64", pool_bytes=128 << 20)
+    pairs = []
+    for i in (0, 2):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        kw.pop("adaptive")
+        m = batch.case(z, h0, 10.0, **kw)
+        m.set_rain(rain, 60.0)
+        o = po.Oracle(z, h0, 10.0, dt=kw["dt"], manning=kw["manning"])
+        o.set_rain(rain, 60.0)
+        batch.attach(m, max_iters=90)
+        pairs.append((m, o))
+    while batch.step(12):
+        pass
+    for m, o in pairs:
+        o.step(90)
+        for f in ("h", "um", "vn", "uu", "vv", "uua", "vva"):
+            assert np.array_equal(m.field(f), o.field(f)), f
+    batch.close()
+
+
+def test_batch_time_bound(pkg, cases):
+    """t_end stops a member where run_until stops a stand-alone handle."""
+    z, h0, kw, rain, outlet = scenario(cases, 1)   # adaptive
+    a = pkg.Case(z, h0, 10.0, precision="fp32", tile_rows=32, **kw)
+    a.set_rain(rain, 60.0)
+    n_alone = a.run_until(95.0)
+    batch = pkg.Batch(precision="fp32", pool_bytes=64 << 20, tile_rows=32)
+    m = batch.case(z, h0, 10.0, **kw)
+    m.set_rain(rain, 60.0)
+    batch.attach(m, t_end=95.0)
+    while batch.step(6):
+        pass
+    done, n, t = batch.status(m)
+    assert done and n == n_alone and t == a.time and t <= 95.0
+    assert_same(m, a, "t_end")
+    batch.close()
+
+
+def test_pool_admission(pkg, cases):
+    """A member that does not fit the pool is refused with SW2D_E_NOMEM; it fits
+    once another member is destroyed; the footprint bounds the pool use."""
+    R, C = 300, 300
+    fp = pkg.footprint(R, C, precision="fp32", batch_tile_rows=32, hydro_cap=100)
+    batch = pkg.Batch(precision="fp32", pool_bytes=int(fp * 1.6), tile_rows=32)
+    z = dem(cases, R, C, 5)
+    h0 = np.zeros((R, C))
+    a = batch.case(z, h0, 10.0, dt=0.5, manning=0.03)
+    a.set_hydrograph(100)
+    used, _, cap = batch.memory()
+    assert used <= fp
+    with pytest.raises(pkg.Sw2dError, match=r"\(-3\)"):
+        batch.case(z, h0, 10.0, dt=0.5, manning=0.03)
+    a.close()
+    assert batch.memory()[0] == 0
+    b = batch.case(z, h0, 10.0, dt=0.5, manning=0.03)
+    batch.attach(b, max_iters=12)
+    assert batch.step(12) == 0
+    batch.close()
+
+
+def test_batch_autotune_keeps_results(pkg, cases):
+    batch = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
+    alone, members = [], []
+    for i in range(3):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision="fp32", **kw)
+        a.set_rain(rain, 60.0)
+        a.step(30)
+        alone.append(a)
+        m = batch.case(z, h0, 10.0, **kw)
+        m.set_rain(rain, 60.0)
+        batch.attach(m, max_iters=30)
+        members.append(m)
+    H = batch.autotune(3)
+    assert H > 0
+    while batch.step(6):
+        pass
+    for i in range(3):
+        assert_same(members[i], alone[i], f"autotuned member {i}")
+    batch.close()

@@ -294,3 +294,50 @@ def test_lake_catchment_smooth_example_mask(po, pkg, cases):
     assert np.array_equal(a, b)
     changed = a != z
     assert changed.sum() > 0 and np.all(mask[changed] != 1)
+
+
+def test_derived_fields_of_linear_state(po):
+    """hm_hn, uu1_vv1, uu_vv, uua_vva (SW2D-EFDC.cu:362-522) on states whose
+    averages have closed forms: a field linear in (row, col) averaged over two
+    or four face points equals its value at their centroid.  Dyadic
+    coefficients keep every average exact, so a wrong neighbour index (a
+    shifted centroid) changes the value.  Boundary rows/columns use the
+    one-sided forms of the reference."""
+    R, C = 9, 11
+    r = np.arange(R, dtype=np.float64)[:, None] + np.zeros((1, C))
+    j = np.arange(C, dtype=np.float64)[None, :] + np.zeros((R, 1))
+    z = np.zeros((R, C))
+    o = po.Oracle(z, np.ones((R, C)), 10.0, dt=0.5, manning=0.03)
+    # depth linear in (r, j): face depths at the face centres
+    h = 1.0 + 0.5 * r + 0.25 * j
+    o.set_state(h, np.zeros((R, C)), np.zeros((R, C)))
+    hm, hn = o.field("hm"), o.field("hn")
+    assert np.array_equal(hm[1:], (1.0 + 0.5 * (r - 0.5) + 0.25 * j)[1:])
+    assert np.array_equal(hm[0], h[0])                     # north edge: one-sided
+    assert np.array_equal(hn[:, 1:], (1.0 + 0.5 * r + 0.25 * (j - 0.5))[:, 1:])
+    assert np.array_equal(hn[:, 0], h[:, 0])               # west edge: one-sided
+    # unit depth, fluxes linear in (r, j): velocities equal the fluxes
+    a, b, c0 = 0.25, 0.125, 1.0
+    um = a * r + b * j + c0
+    vn = -0.5 * r + 0.375 * j - 2.0
+    o.set_state(np.ones((R, C)), um, vn)
+    uu1, vv1 = o.field("uu1"), o.field("vv1")
+    assert np.array_equal(uu1, um) and np.array_equal(vv1, vn)
+    uua, vva, uu, vv = o.field("uua"), o.field("vva"), o.field("uu"), o.field("vv")
+    # uua: four row faces around the column face (r + 1/2, j - 1/2)
+    assert np.array_equal(uua[:-1, 1:], (a * (r + 0.5) + b * (j - 0.5) + c0)[:-1, 1:])
+    assert np.array_equal(uua[:-1, 0], (a * (r + 0.5) + c0)[:-1, 0])
+    # vva: four column faces around the row face (r - 1/2, j + 1/2)
+    lin_v = lambda rr, jj: -0.5 * rr + 0.375 * jj - 2.0  # noqa: E731
+    assert np.array_equal(vva[1:, :-1], lin_v(r - 0.5, j + 0.5)[1:, :-1])
+    assert np.array_equal(vva[0, :-1], lin_v(r, j + 0.5)[0, :-1])
+    # cell centres
+    assert np.array_equal(uu[:-1], (a * (r + 0.5) + b * j + c0)[:-1])
+    assert np.array_equal(vv[:, :-1], lin_v(r, j + 0.5)[:, :-1])
+    # the south row's uua and uu see the (zero) pad row below the grid
+    assert np.array_equal(uu[-1], 0.5 * um[-1])
+    # a dry cell (h < th) has zero centre velocity
+    hd = np.ones((R, C))
+    hd[4, 5] = 0.5e-4
+    o.set_state(hd, um, vn)
+    assert o.field("uu")[4, 5] == 0.0 and o.field("vv")[4, 5] == 0.0

@@ -1,0 +1,95 @@
+"""Serving: continuous batching with HBM-footprint admission.
+
+Each request's result (outlet hydrograph and volume ledger) must be the same
+alone and in the mix: identical, value for value, to the same scenario run on
+a stand-alone handle with the same tile rows, whatever ran beside it and
+however long it queued.  The mix here arrives staggered while earlier
+requests run, and the pool holds only part of it at once, so requests are
+admitted as memory frees up.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+H = 32
+CAP = 4096   # hydrograph records kept per request
+
+
+def request(cases, i):
+    shapes = [(64, 90), (150, 120), (48, 300), (200, 200), (96, 96), (130, 70)]
+    R, C = shapes[i % len(shapes)]
+    rng = np.random.default_rng(500 + i)
+    slope = rng.uniform(0.002, 0.03)
+    r = np.arange(R, dtype=np.float64)[:, None]
+    z = slope * 10.0 * (R - 1 - r) + cases.fbm(R, C, 0.7, rng.uniform(0.5, 5.0), 500 + i)
+    z = np.broadcast_to(z, (R, C)).copy()
+    outlet = (R - 1, int(np.argmin(z[R - 1])))
+    rain = np.array([rng.uniform(10, 120) * 0.25, 0.0, 0.0, 0.0])   # mm per 15 min
+    kw = dict(dt=0.5, manning=rng.uniform(0.02, 0.1), adaptive=bool(i % 3 == 2))
+    t_end = 600.0 + 60.0 * i
+    return z, outlet, rain, kw, t_end
+
+
+def alone(pkg, z, outlet, rain, kw, t_end):
+    c = pkg.Case(z, np.zeros_like(z), 10.0, precision="fp32", tile_rows=H, **kw)
+    c.set_rain(rain, 900.0)
+    c.set_outlet(*outlet)
+    c.set_hydrograph(CAP)
+    c.run_until(t_end)
+    out = c.hydrograph(), c.balance()
+    c.close()
+    return out
+
+
+def test_results_same_alone_and_in_the_mix(pkg, cases):
+    n = 6
+    reqs = [request(cases, i) for i in range(n)]
+    fps = [pkg.footprint(z.shape[0], z.shape[1], precision="fp32", batch_tile_rows=H,
+                         hydro_cap=CAP) for z, _, _, _, _ in reqs]
+    # room for about half of the mix at once
+    srv = pkg.Server(precision="fp32", pool_bytes=int(sum(sorted(fps)[-3:]) * 1.05), tile_rows=H,
+                     slice_iters=12)
+    ids = []
+    for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
+        ids.append(srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                              hydro_cap=CAP, **kw))
+        if i == 0:
+            time.sleep(0.005)   # the first one starts alone; the others arrive while it runs
+    srv.wait(timeout=300)
+    st = srv.stats()
+    assert st["completed"] == n and st["failed"] == 0 and st["queued"] == 0
+    assert st["pool_peak"] <= st["pool_capacity"]
+    res = [srv.poll(i) for i in ids]
+    # overlap: some request was admitted before another one had finished
+    assert any(a["t_admit"] < b["t_done"] and b["t_admit"] < a["t_done"]
+               for a in res for b in res if a is not b)
+    # the pool does not hold the whole mix: some request waited for a retirement
+    assert any(a["t_admit"] >= b["t_done"] for a in res for b in res if a is not b)
+    for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
+        r = res[i]
+        assert r["state"] == "done"
+        hq, bal = alone(pkg, z, outlet, rain, kw, t_end)
+        got = srv.hydrograph(ids[i])
+        assert np.array_equal(got, hq), f"request {i} hydrograph differs from the stand-alone run"
+        assert r["balance"] == bal, f"request {i} ledger differs"
+        assert r["time"] <= t_end
+        srv.release(ids[i])
+    assert st["p99_latency"] >= st["p50_latency"] > 0
+    srv.close()
+
+
+def test_request_too_large_fails_cleanly(pkg):
+    """A request larger than the whole pool fails; the server keeps serving."""
+    big = np.zeros((600, 600)) + np.arange(600)[::-1, None] * 0.1   # ~10 MB of FP32 state
+    srv = pkg.Server(precision="fp32", pool_bytes=2 << 20, tile_rows=H)
+    rid = srv.submit(big, 10.0, t_end=60.0, dt=0.5, manning=0.03)
+    srv.wait(rid, timeout=60)
+    assert srv.poll(rid)["state"] == "failed"
+    small = np.zeros((8, 8)) + np.arange(8)[::-1, None] * 0.1
+    rid2 = srv.submit(small, 10.0, t_end=30.0, dt=0.5, manning=0.03)
+    srv.wait(rid2, timeout=60)
+    assert srv.poll(rid2)["state"] == "done"
+    srv.close()
