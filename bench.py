@@ -1,19 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the SW2D time step on one MI355X.
 
-Workload (default): the 2048 x 2048 "plain FP64 GPU drop-in at real size"
-case -- dx = 5 m, 0.5 % slope + 3 Gaussian-valley channels + fBm 3 m +
-prefilled Gaussian lake pit whose rim cells are the EFDC inflow records,
+Workload of the headline line: the 2048 x 2048 "plain FP64 GPU drop-in at
+real size" case -- dx = 5 m, 0.5 % slope + 3 Gaussian-valley channels + fBm
+3 m + prefilled Gaussian lake pit whose rim cells are the EFDC inflow records,
 Manning 0.02 (channels) / 0.08 (hillslopes), moving Gaussian storm (sigma
 1.5 km, 80 mm/h, 5 m/s, 1 h), CFL-adaptive step (Courant 0.5) -- run through
-the fused FP64 kernel.  One step = one loop iteration of the reference
-(flux, continuity, treat_error, continuity, derived fields) over all cells;
-the metric is cell-updates per second with the state resident in HBM.
+the fused FP64 kernel.  One step = one loop iteration of the reference (flux,
+continuity, treat_error, continuity, derived fields) over all cells; the
+metric is cell-updates per second with the state resident in HBM.
 
-Printed: one JSON line with the value, the roofline of the step kernel
-(algorithmic bytes per launch / average launch time from HIP events on the
-library's stream) and the CPU baseline (the FP64 oracle, OpenMP, on a bounded
-sample of the same workload).
+Also measured (same JSON line): the FP32-state / FP64-balance form on the same
+case; the plain FP64 kernel sequence; a batch of 256 ensemble scenarios
+(128..1024 square grids, 1 h each) stepped together; a bounded serving
+session (Poisson arrivals, grids 256..4096, 2 h storms) with p50/p99
+time-to-hydrograph; the plain FP64 path's time for the same scenarios one at
+a time; the CPU baseline (the FP64 oracle, OpenMP, on a bounded sample).
 """
 import argparse
 import importlib.util
@@ -43,6 +45,11 @@ def load_package():
     return mod
 
 
+def log(msg):
+    """Progress on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def bytes_per_cell(precision, has_class_map):
     """Compulsory HBM bytes of one cell-update of the fused kernel: read h, um,
     vn and the FP64 bed z, write h, um, vn (+1 byte Manning class when a class
@@ -67,6 +74,181 @@ def make_case(pkg, spec, precision, kernel, tile_rows):
     if spec.lake_mask is not None:
         c.set_lake_bc(spec.lake_mask)
     return c
+
+
+def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps):
+    """Autotune (tile_rows 0), warm up, then time `steps` iterations with HIP
+    events on the library's stream around every launch (or graph replay)."""
+    case = make_case(pkg, spec, precision, kernel, tile_rows)
+    if kernel == "fused" and tile_rows == 0:
+        tile_rows = case.autotune(10)
+    case.step(warmup)
+    case.sync()
+    case.timing(True)
+    case.sync()
+    t0 = time.perf_counter()
+    case.step(steps)
+    case.sync()
+    wall = time.perf_counter() - t0
+    kms, launches = case.kernel_time()
+    case.timing(False)
+    R, C = spec.z.shape
+    t_sim, _, dt2 = case.clock()
+    bpc = bytes_per_cell(precision, spec.nclass is not None)
+    avg_s = (kms / 1e3) / max(launches, 1)
+    return case, dict(value=R * C * steps / wall, ms_per_step=1e3 * wall / steps,
+                      tile_rows=tile_rows, t_sim=t_sim, dt2=dt2, balance=case.balance(), bpc=bpc,
+                      avg_s=avg_s, achieved=bpc * R * C / avg_s / 1e9,
+                      ws_mb=case.device_bytes() / 1e6, device_bytes=case.device_bytes())
+
+
+def plain_rate(pkg, side, rates, iters=30):
+    """cell-updates/s of the plain FP64 kernel sequence on one side x side
+    ensemble scenario (cached per side)."""
+    if side in rates:
+        return rates[side]
+    cases = sys.modules["sw2d_efdc_amd.cases"]
+    spec = cases.ensemble_scenario(0, side=side)
+    c = make_case(pkg, spec, "fp64", "plain", 0)
+    c.step(3)
+    c.sync()
+    t0 = time.perf_counter()
+    c.step(iters)
+    c.sync()
+    rates[side] = side * side * iters / (time.perf_counter() - t0)
+    c.close()
+    return rates[side]
+
+
+def plain_time(pkg, cells_iters, rates):
+    """Seconds the plain FP64 path takes for (side, iterations) scenarios run one
+    at a time, from its measured rate at the nearest measured side."""
+    total = 0.0
+    for side, iters in cells_iters:
+        key = min((128, 256, 512, 1024, 2048, 4096), key=lambda s: abs(np.log(s / side)))
+        total += side * side * iters / plain_rate(pkg, key, rates)
+    return total
+
+
+def batch_leg(pkg, cases, n, plain_rates):
+    """The ensemble: n scenarios (128..1024, 1 h, fixed dt 0.5 s) in one batch,
+    FP32 state / FP64 balance, tile rows by sw2d_batch_autotune."""
+    specs = [cases.ensemble_scenario(k) for k in range(n)]
+    log(f"batch: {n} scenarios generated")
+    need = sum(pkg.footprint(s.z.shape[0], s.z.shape[1], precision="fp32", batch_tile_rows=8,
+                             hydro_cap=int(s.t_end / (2 * s.dt)) + 16) for s in specs)
+    b = pkg.Batch(precision="fp32", pool_bytes=int(need * 1.1) + (64 << 20))
+    members = []
+    t_setup = time.perf_counter()
+    for s in specs:
+        m = b.case(s.z, s.h0, s.cellsize, dt=s.dt, manning=s.manning)
+        m.set_rain(s.rain, s.dtrain)
+        m.set_outlet(*s.outlet)
+        m.set_hydrograph(int(s.t_end / (2 * s.dt)) + 16)
+        b.attach(m, t_end=s.t_end)
+        members.append(m)
+    t_setup = time.perf_counter() - t_setup
+    H = b.autotune(5)
+    b.timing(True)
+    t0 = time.perf_counter()
+    while b.step(60):
+        pass
+    wall = time.perf_counter() - t0
+    kms, launches = b.kernel_time()
+    iters = [b.status(m)[1] for m in members]
+    cell_updates = sum(s.cells * k for s, k in zip(specs, iters))
+    bytes_alg = sum(s.cells * k for s, k in zip(specs, iters)) * bytes_per_cell("fp32", False)
+    achieved = bytes_alg / (kms / 1e3) / 1e9
+    vol_err = max(abs(m.balance()["error"]) / max(m.balance()["volume"], 1e-30) for m in members)
+    q_peak = [m.balance()["q_peak"] for m in members]
+    t_plain = plain_time(pkg, [(s.z.shape[0], k) for s, k in zip(specs, iters)], plain_rates)
+    b.close()
+    return {
+        "workload": f"{n} ensemble scenarios, square grids 128..1024 (log-uniform), dx 10 m, "
+                    "slope U(0.2%,3%), fBm U(0.5,5) m, n U(0.02,0.1), block rain U(10,120) mm/h "
+                    "for U(15,90) min, free outlet, fixed dt 0.5 s, 1 h each, one launch per "
+                    "iteration for all",
+        "precision": "f32 state / f64 balance",
+        "cells_total": int(sum(s.cells for s in specs)),
+        "tile_rows": H,
+        "iterations": int(max(iters)),
+        "wall_s": wall,
+        "setup_s": t_setup,
+        "value": cell_updates / wall,
+        "unit": "cell-updates/s",
+        "scenarios_per_s": n / wall,
+        "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell_update": 32,
+                     "kernel_ms_total": kms, "launches": launches},
+        "max_volume_error_rel": vol_err,
+        "mean_q_peak_m3s": float(np.mean(q_peak)),
+        "plain_fp64_one_at_a_time_s": t_plain,
+        "scenario_throughput_vs_plain_fp64": t_plain / wall,
+    }
+
+
+def serve_leg(pkg, cases, seconds, rate, hours, plain_rates):
+    """A bounded serving session: Poisson arrivals at `rate`/s for `seconds`,
+    grid sides 256..4096 (p 0.4/0.3/0.15/0.1/0.05), 2 h storms, FP32 form."""
+    n = max(1, int(round(seconds * rate)))
+    arrivals, specs = cases.serve_requests(n, rate=rate, t_end=3600.0 * hours)
+    log(f"serving: {n} requests generated; submitting")
+    tile = 32
+    srv = pkg.Server(precision="fp32", pool_bytes=64 << 30, tile_rows=tile, slice_iters=60)
+    cap = int(3600.0 * hours / (2 * 0.5)) + 16
+    ids = []
+    t0 = time.perf_counter()
+    for a, s in zip(arrivals, specs):
+        d = a - (time.perf_counter() - t0)
+        if d > 0:
+            time.sleep(d)
+        ids.append(srv.submit(s.z, s.cellsize, t_end=s.t_end, rain=s.rain, dtrain=s.dtrain,
+                              outlet=s.outlet, hydro_cap=cap, dt=s.dt, manning=s.manning))
+    log("serving: all submitted")
+    while True:
+        try:
+            srv.wait(timeout=30.0)
+            break
+        except pkg.Sw2dError:
+            st = srv.stats()
+            log(f"serving: {st['completed']} done, {st['running']} running, {st['queued']} queued")
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    res = [srv.poll(i) for i in ids]
+    iters = [r["iterations"] for r in res]
+    vol_err = max(abs(r["balance"]["error"]) / max(r["balance"]["volume"], 1e-30) for r in res)
+    t_plain = plain_time(pkg, [(s.z.shape[0], k) for s, k in zip(specs, iters)], plain_rates)
+    bytes_alg = st["cell_updates"] * bytes_per_cell("fp32", False)
+    achieved = bytes_alg / (st["kernel_ms"] / 1e3) / 1e9 if st["kernel_ms"] > 0 else None
+    srv.close()
+    sides = [s.z.shape[0] for s in specs]
+    return {
+        "workload": f"{n} requests, Poisson {rate}/s over {seconds} s, sides "
+                    f"{sorted(set(sides))} (p 0.4/0.3/0.15/0.1/0.05), {hours} h storms, dx 10 m, "
+                    "fixed dt 0.5 s, FP32 state / FP64 balance, HBM-footprint admission",
+        "requests": n,
+        "completed": st["completed"],
+        "failed": st["failed"],
+        "cells_mean": float(np.mean([s.cells for s in specs])),
+        "wall_s": wall,
+        "value": st["cell_updates"] / wall,
+        "unit": "cell-updates/s",
+        "busy_s": st["busy_seconds"],
+        "p50_time_to_hydrograph_s": st["p50_latency"],
+        "p99_time_to_hydrograph_s": st["p99_latency"],
+        "max_time_to_hydrograph_s": st["max_latency"],
+        "p50_queue_s": st["p50_wait"],
+        "p99_queue_s": st["p99_wait"],
+        "pool_peak_bytes": st["pool_peak"],
+        "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "bytes_per_cell_update": 32},
+        "max_volume_error_rel": vol_err,
+        "offered_cell_updates_per_s": float(sum(s.cells * k for s, k in zip(specs, iters))
+                                            / max(arrivals[-1], 1e-9)),
+        "plain_fp64_one_at_a_time_s": t_plain,
+        "scenario_throughput_vs_plain_fp64": t_plain / wall,
+    }
 
 
 def cpu_baseline(spec, seconds, threads):
@@ -119,11 +301,20 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0,
                     help="rows per tile of the fused kernel (0 = measured by sw2d_autotune)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
-                    help="after the run, copy this many MB device-to-device with torch (a plain "
-                         "copy kernel of known size, to scale rocprofv3 FETCH/WRITE_SIZE)")
+                    help="after the run, stream this many MB through the library's probe kernel "
+                         "(reads and writes n doubles: a copy of known size, to scale rocprofv3 "
+                         "FETCH/WRITE_SIZE)")
     ap.add_argument("--plain-steps", type=int, default=200,
                     help="iterations of the plain FP64 kernel sequence for the comparison "
                          "line (0 = skip)")
+    ap.add_argument("--fp32-steps", type=int, default=3000,
+                    help="iterations of the FP32 form on the same case (0 = skip)")
+    ap.add_argument("--batch-scenarios", type=int, default=256,
+                    help="scenarios of the batched ensemble leg (0 = skip)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0,
+                    help="length of the Poisson arrival window of the serving leg (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-hours", type=float, default=2.0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -140,34 +331,13 @@ def main():
     R, C = spec.z.shape
     cells = R * C
 
-    case = make_case(pkg, spec, args.precision, args.kernel, args.tile_rows)
-    tile_rows = args.tile_rows
-    if args.kernel == "fused" and tile_rows == 0:
-        tile_rows = case.autotune(10)
-    case.step(args.warmup)
-    case.sync()
-
-    # timed region: K iterations, HIP events around every launch on the library stream
-    case.timing(True)
-    case.sync()
-    t0 = time.perf_counter()
-    case.step(args.steps)
-    case.sync()
-    t1 = time.perf_counter()
-    kms, launches = case.kernel_time()
-    case.timing(False)
-    wall = t1 - t0
-    value = cells * args.steps / wall
-    ms_per_step = 1e3 * wall / args.steps
-    t_sim, mstep, dt2 = case.clock()
-    bal = case.balance()
-
-    bpc = bytes_per_cell(args.precision, spec.nclass is not None)
-    per_launch = bpc * cells
-    avg_s = (kms / 1e3) / max(launches, 1)
-    achieved = per_launch / avg_s / 1e9
-    ws_mb = case.device_bytes() / 1e6
-
+    log(f"{R}x{C} case ready; timing {args.precision} {args.kernel}")
+    case, r = timed_run(pkg, spec, args.precision, args.kernel, args.tile_rows, args.warmup,
+                        args.steps)
+    log(f"{args.precision}: {r['value']:.3e} cell-updates/s, tile rows {r['tile_rows']}")
+    value, ms_per_step, tile_rows = r["value"], r["ms_per_step"], r["tile_rows"]
+    t_sim, dt2, bal, bpc, achieved, avg_s, ws_mb = (r["t_sim"], r["dt2"], r["balance"], r["bpc"],
+                                                     r["achieved"], r["avg_s"], r["ws_mb"])
     out = {
         "metric": METRIC,
         "value": value,
@@ -191,7 +361,7 @@ def main():
             "tile_rows": tile_rows,
             "simulated_s_after_run": t_sim,
             "dt2_last_s": dt2,
-            "device_bytes": case.device_bytes(),
+            "device_bytes": r["device_bytes"],
         },
         "roofline": {
             "bound": "hbm",
@@ -207,8 +377,22 @@ def main():
         "volume_error_rel": bal["error"] / max(bal["volume"], 1e-30),
     }
 
+    case.close()
+    del case
+    if args.fp32_steps > 0 and args.kernel == "fused" and args.precision == "fp64":
+        c32, r32 = timed_run(pkg, spec, "fp32", "fused", 0, args.warmup, args.fp32_steps)
+        c32.close()
+        log(f"fp32: {r32['value']:.3e} cell-updates/s, tile rows {r32['tile_rows']}")
+        out["fp32_form"] = {
+            "value": r32["value"], "unit": "cell-updates/s", "tile_rows": r32["tile_rows"],
+            "steps": args.fp32_steps, "ms_per_step": r32["ms_per_step"],
+            "roofline": {"achieved": r32["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": r32["achieved"] / HBM_PEAK_GBS,
+                         "bytes_per_cell_update": r32["bpc"],
+                         "kernel_avg_us": r32["avg_s"] * 1e6},
+            "volume_error_rel": r32["balance"]["error"] / max(r32["balance"]["volume"], 1e-30),
+        }
     if args.plain_steps > 0 and args.kernel == "fused" and args.precision == "fp64":
-        del case
         plain = make_case(pkg, spec, "fp64", "plain", 0)
         plain.step(5)
         plain.sync()
@@ -217,13 +401,25 @@ def main():
         plain.sync()
         pw = time.perf_counter() - t0
         pv = cells * args.plain_steps / pw
+        log(f"plain fp64: {pv:.3e} cell-updates/s")
         out["plain_fp64"] = {"value": pv, "unit": "cell-updates/s",
                              "steps": args.plain_steps,
                              "fused_speedup": value / pv}
         del plain
 
+    plain_rates = {}
+    if args.batch_scenarios > 0:
+        out["batch"] = batch_leg(pkg, cases, args.batch_scenarios, plain_rates)
+        log(f"batch: {out['batch']['value']:.3e} cell-updates/s")
+    if args.serve_seconds > 0:
+        out["serving"] = serve_leg(pkg, cases, args.serve_seconds, args.serve_rate,
+                                   args.serve_hours, plain_rates)
+        log(f"serving: p50 {out['serving']['p50_time_to_hydrograph_s']:.2f} s, "
+            f"p99 {out['serving']['p99_time_to_hydrograph_s']:.2f} s")
+
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline: {threads} threads, {args.cpu_seconds} s")
         out["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, threads)
     else:
         out["cpu_baseline"] = None

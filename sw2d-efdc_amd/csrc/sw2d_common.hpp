@@ -12,9 +12,9 @@ struct StepTime {
 };
 
 // start time and length of this iteration from the clock ring
-__device__ __forceinline__ StepTime step_time(const StepCommon &s)
+__device__ __forceinline__ StepTime step_time(const StepCommon &s, const Iter &it)
 {
-    const Clock &ck = s.clk[s.slot_a];
+    const Clock &ck = s.clk[it.slot_a];
     StepTime st;
     st.t = ck.t;
     if (s.fixed_dt) {
@@ -159,7 +159,8 @@ __device__ __forceinline__ void reduce_partials(const Partial *p, int n, int lan
 }
 
 template <typename F>
-__device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &vel, const int part,
+__device__ void bookkeeping(const StepCommon &s, const Iter &it, const StepTime &st, const F &vel,
+                            const int part,
                             const int lane)
 {
     if (part > 0) {
@@ -173,18 +174,18 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
         return;
     }
     if (lane == 0) {
-        Clock &nb = s.clk[s.slot_b];
+        Clock &nb = s.clk[it.slot_b];
         nb.t = st.tm + st.dt;
         nb.dt2_prev = st.dt2;
-        nb.mstep = s.clk[s.slot_a].mstep + 2;
-        s.clk[s.slot_c].cfl_bits = kInfBits;
+        nb.mstep = s.clk[it.slot_a].mstep + 2;
+        s.clk[it.slot_c].cfl_bits = kInfBits;
     }
-    if (s.step_index > 0) {
+    if (it.step_index > 0) {
         double vol, src, clip, qo;
-        reduce_partials(s.part_in, s.nparts, lane, vol, src, clip, qo);
+        reduce_partials(it.part_in, s.nparts, lane, vol, src, clip, qo);
         if (lane == 0) {
             Ledger &L = *s.ledger;
-            double dt2p = s.clk[s.slot_a].dt2_prev;
+            double dt2p = s.clk[it.slot_a].dt2_prev;
             L.v_src += src * s.dx * s.dy;
             L.v_clip += clip * s.dx * s.dy;
             L.v_out += qo * dt2p;
@@ -192,7 +193,7 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
             if (qo > L.q_peak) { L.q_peak = qo; L.t_peak = st.t; }
             L.vol_last = vol * s.dx * s.dy;
             L.reduced += 1;
-            long long i = s.step_index - 1;
+            long long i = it.step_index - 1;
             if (s.hydro && i < s.hydro_cap) {
                 s.hydro[2 * i] = st.t;
                 s.hydro[2 * i + 1] = qo;
@@ -205,13 +206,14 @@ __device__ void bookkeeping(const StepCommon &s, const StepTime &st, const F &ve
 // last iteration, which the next iteration would reduce.
 __device__ __forceinline__ void ledger_peek(const StepCommon &s, Ledger *out, const int lane)
 {
+    const Iter it = iter_of(s);
     double vol = 0, src = 0, clip = 0, qo = 0;
-    if (s.step_index > 0) reduce_partials(s.part_in, s.nparts, lane, vol, src, clip, qo);
+    if (it.step_index > 0) reduce_partials(it.part_in, s.nparts, lane, vol, src, clip, qo);
     if (lane == 0) {
         Ledger L = *s.ledger;
-        if (s.step_index > 0) {
-            double dt2p = s.clk[s.slot_a].dt2_prev;
-            double t = s.clk[s.slot_a].t;
+        if (it.step_index > 0) {
+            double dt2p = s.clk[it.slot_a].dt2_prev;
+            double t = s.clk[it.slot_a].t;
             L.v_src += src * s.dx * s.dy;
             L.v_clip += clip * s.dx * s.dy;
             L.v_out += qo * dt2p;
@@ -219,7 +221,7 @@ __device__ __forceinline__ void ledger_peek(const StepCommon &s, Ledger *out, co
             if (qo > L.q_peak) { L.q_peak = qo; L.t_peak = t; }
             L.vol_last = vol * s.dx * s.dy;
             L.reduced += 1;
-            long long i = s.step_index - 1;
+            long long i = it.step_index - 1;
             if (s.hydro && i < s.hydro_cap) {
                 s.hydro[2 * i] = t;
                 s.hydro[2 * i + 1] = qo;

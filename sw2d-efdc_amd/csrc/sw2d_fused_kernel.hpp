@@ -66,20 +66,23 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         sid = t.sid;
         tile = t.local;
     }
+    // the descriptor is read in place (no local copy: its uniform fields stay
+    // reloadable scalar loads instead of pinned registers)
     const Scenario &d = scen[sid];
-    StepCommon s = d.s;
+    const StepCommon &s = d.s;
     // iteration k of this scenario: k mod 6 from the launch phase, k itself
     // from the clock slot the previous iteration wrote
     const int k6 = (phase - d.phase0 + 6) % 6;
-    s.slot_a = k6 % 3; s.slot_b = (k6 + 1) % 3; s.slot_c = (k6 + 2) % 3;
-    s.part_out = d.part + (size_t)(k6 & 1) * s.nparts;
-    s.part_in = d.part + (size_t)((k6 + 1) & 1) * s.nparts;
-    s.step_index = s.clk[s.slot_a].mstep >> 1;
-    const StepTime st = step_time(s);
+    Iter it;
+    it.slot_a = k6 % 3; it.slot_b = (k6 + 1) % 3; it.slot_c = (k6 + 2) % 3;
+    it.part_out = d.part + (size_t)(k6 & 1) * s.nparts;
+    it.part_in = d.part + (size_t)((k6 + 1) & 1) * s.nparts;
+    it.step_index = s.clk[it.slot_a].mstep >> 1;
+    const StepTime st = step_time(s, it);
     if (d.status) {
         // batch member: a finished scenario leaves its blocks idle
         if (d.status->done) return;
-        if (!(s.step_index < d.k_end && st.t + st.dt <= d.t_end)) {
+        if (!(it.step_index < d.k_end && st.t + st.dt <= d.t_end)) {
             if (tile == d.ntiles && lane == 0) d.status->done = 1;
             return;
         }
@@ -106,9 +109,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             va = (double)ps.vva(r, j);
         };
         const int part = tile - d.ntiles;
-        bookkeeping(s, st, vel, part, lane);
+        bookkeeping(s, it, st, vel, part, lane);
         if (part == 0 && lane == 0 && d.status) {
-            const long long kn = s.step_index + 1;
+            const long long kn = it.step_index + 1;
             const double tn = st.tm + st.dt;
             d.status->k = kn;
             d.status->t = tn;
@@ -131,6 +134,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // CFL candidates only for an adaptive time step; a wave-uniform flag, so
     // a scenario runs the same instructions alone and in a mixed batch
     const bool adapt = !s.fixed_dt;
+    // FP32 form: water-surface tests from bed differences carried per face
+    // (z stays FP64 only to form them), storm factors and source terms in FP32
+    constexpr bool kF32 = sizeof(T) == 4;
     const T th = (T)s.th, gg = (T)s.g, dx = (T)s.dx, dy = (T)s.dy;
     const T rdx = (T)(1.0 / s.dx), rdy = (T)(1.0 / s.dy);
     const T dt2 = (T)st.dt2;
@@ -142,19 +148,23 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const double ev1 = evap_rate(s, st.t), ev2 = evap_rate(s, st.tm);
     const bool storm1 = storm_live(s, st.t), storm2 = storm_live(s, st.tm);
     double gx1 = 0, gx2 = 0;
-    __shared__ double gy_s[2][kMaxTileRows + 8];
+    T pk1 = 0, pk2 = 0;  // FP32 form: peak * gx of this column, 0 outside the storm window
+    __shared__ T gy_s[2][kMaxTileRows + 8];
     if (s.storm_on) {
         const double x = ((double)j + 0.5) * s.dx;
         gx1 = storm_factor(x - (s.st_x0 + s.st_vx * st.t), s.st_sigma);
         gx2 = storm_factor(x - (s.st_x0 + s.st_vx * st.tm), s.st_sigma);
+        pk1 = storm1 ? (T)(s.st_peak * gx1) : T(0);
+        pk2 = storm2 ? (T)(s.st_peak * gx2) : T(0);
         for (int i = lane; i < d.H + 6; i += 64) {
             const double y1 = ((double)(r0 - 5 + i) + 0.5) * s.dy;
             const double y2 = ((double)(r0 - 6 + i) + 0.5) * s.dy;
-            gy_s[0][i] = storm_factor(y1 - (s.st_y0 + s.st_vy * st.t), s.st_sigma);
-            gy_s[1][i] = storm_factor(y2 - (s.st_y0 + s.st_vy * st.tm), s.st_sigma);
+            gy_s[0][i] = (T)storm_factor(y1 - (s.st_y0 + s.st_vy * st.t), s.st_sigma);
+            gy_s[1][i] = (T)storm_factor(y2 - (s.st_y0 + s.st_vy * st.tm), s.st_sigma);
         }
         __syncthreads();
     }
+    const T rr1t = (T)rr1, rr2t = (T)rr2, ev1t = (T)ev1, ev2t = (T)ev2;
     const T P_lake = (T)s.P_lake, P_land = (T)s.P_land, ewl = (T)s.EV_WL_min;
 
     // outlet
@@ -169,6 +179,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     T uu1_1 = 0, uu1_2 = 0, uu1_1w = 0;
     T vv1_1 = 0, vv1_2 = 0, vv1_1e = 0, vv1_2e = 0, vv1_1w = 0;
     T hm1 = 0, hn1 = 0, vva1 = 0;
+    T dzn1 = 0, dzw1 = 0;  // FP32 form: z(k-2) - z(k-1) and z(k-1, j-1) - z(k-1, j)
     T nc1 = rn, nc2 = rn, nc1w = rn;
     T umnP = 0, vnnP = 0, umpP = 0, vnpP = 0;
     bool dryP = false;
@@ -176,7 +187,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     double acc_vol = 0, acc_src = 0, acc_clip = 0, acc_q = 0;
     // CFL: the largest |u| + sqrt(g h) per direction; min_f dx / s_f = dx / max_f s_f
     // because rounding dx / s is monotone in s
-    double smax_x = 0.0, smax_y = 0.0;
+    T smax_x = 0, smax_y = 0;
 
     // software pipeline: row k+1 is loaded while row k is processed.  Rows
     // outside the grid load a clamped row and are then replaced by the ghost
@@ -206,7 +217,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
-        const T nc0 = s.ncls_on ? (T)d.s.ntab[cl0] : rn;
+        const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
         const T h0w = from_west(h0);
@@ -215,6 +226,11 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const T u0e = from_east(u0), u0w = from_west(u0);
         const T v0e = from_east(v0), v0w = from_west(v0);
         const T nc0w = from_west(nc0);
+        T dzn0 = 0, dzw0 = 0;
+        if constexpr (kF32) {
+            dzn0 = (T)(z1 - z0);   // bed step across the north face of row k
+            dzw0 = (T)(z0w - z0);  // bed step across the west face of row k
+        }
         const T hn0 = (j == 0) ? h0 : T(0.50) * (h0 + h0w);
         const T hm0 = (k == 0) ? h0 : ((k == R) ? h1 : T(0.50) * (h0 + h1));
         const T uu1_0 = (hm0 >= th) ? M::div_(u0, hm0) : T(0.00);
@@ -230,11 +246,11 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         if (adapt && colout && k >= r0 && k < r1) {
             // CFL candidates of the incoming state on the faces of row k
             if (k >= 1 && a0 && a1 && hm0 > th) {
-                const double w = (double)M::abs_(uu1_0) + sqrt(s.g * (double)hm0);
+                const T w = M::abs_(uu1_0) + M::sqrt_(gg * hm0);
                 smax_x = (w > smax_x) ? w : smax_x;
             }
             if (j >= 1 && a0 && a0w && hn0 > th) {
-                const double w = (double)M::abs_(vv1_0) + sqrt(s.g * (double)hn0);
+                const T w = M::abs_(vv1_0) + M::sqrt_(gg * hn0);
                 smax_y = (w > smax_y) ? w : smax_y;
             }
         }
@@ -249,9 +265,24 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // momentum update) runs only when a lane of the wave needs it.
         const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th));
         if (__any(act_x)) {
-            const double hhe = h1 + z1, hhw = h2 + z2;  // water surfaces, FP64
+            // water surfaces: FP64 (the reference's arithmetic) or, in the
+            // FP32 form, bed difference dz = z2 - z1 plus FP32 depths
+            bool b1, b2;
+            T dsurf, hv3;  // (h1 + z1) - (h2 + z2); max(surfaces) - max(beds)
+            if constexpr (kF32) {
+                const T dz = dzn1;
+                b1 = h1 < dz;
+                b2 = !b1 && (h2 < -dz);
+                dsurf = (h1 - h2) - dz;
+                hv3 = (dz <= T(0)) ? M::max_(h1, h2 + dz) : M::max_(h1 - dz, h2);
+            } else {
+                const double hhe = h1 + z1, hhw = h2 + z2;
+                b1 = hhe < z2;
+                b2 = !b1 && (hhw < z1);
+                dsurf = (T)(hhe - h2 - z2);
+                hv3 = (T)(__builtin_fmax(hhe, hhw) - __builtin_fmax(z1, z2));
+            }
             const T hhep = h1 - th, hhwp = h2 - th;
-            const bool b1 = hhe < z2, b2 = !b1 && (hhw < z1);
             const bool b3 = !b1 && !b2 && (hhep * hhwp < 0);
             const bool b4 = act_x && !b1 && !b2 && !b3;
             T coef = T(0.5440), hv = h2, c2 = gg;
@@ -260,13 +291,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             if (b3) {
                 const T sgnm = M::sign_(hhep - hhwp);  // hhan / |hhan|
                 coef = -sgnm * T(0.350);
-                hv = (T)(__builtin_fmax(hhe, hhw) - __builtin_fmax(z1, z2));
+                hv = hv3;
                 c2 = T(2.00) * gg;
                 wet = (h1 > 0) || (h2 > 0);
             }
             T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
-                const T u13 = gg * hm1 * dt2dx * (T)(hhe - h2 - z2);
+                const T u13 = gg * hm1 * dt2dx * dsurf;
                 const T ur = T(0.50) * (uu1_0 + uu1_1);
                 const T ul = T(0.50) * (uu1_1 + uu1_2);
                 const T umr = ur * (u0 + u1) * T(0.50) + M::abs_(ur) * (u1 - u0) * T(0.50);
@@ -292,9 +323,22 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // y direction: vn at the west face of (f, j); s = column j-1, n = column j
         const bool act_y = flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th));
         if (__any(act_y)) {
-            const double hhn = h1 + z1, hhs = h1w + z1w;
+            bool b1, b2;
+            T dsurf, hv3;  // (h1 + z1) - (h1w + z1w); max(surfaces) - max(beds)
+            if constexpr (kF32) {
+                const T dz = dzw1;
+                b1 = h1 < dz;
+                b2 = !b1 && (h1w < -dz);
+                dsurf = (h1 - h1w) - dz;
+                hv3 = (dz <= T(0)) ? M::max_(h1, h1w + dz) : M::max_(h1 - dz, h1w);
+            } else {
+                const double hhn = h1 + z1, hhs = h1w + z1w;
+                b1 = hhn < z1w;
+                b2 = !b1 && (hhs < z1);
+                dsurf = (T)(hhn - h1w - z1w);
+                hv3 = (T)(__builtin_fmax(hhn, hhs) - __builtin_fmax(z1, z1w));
+            }
             const T hhnp = h1 - th, hhsp = h1w - th;
-            const bool b1 = hhn < z1w, b2 = !b1 && (hhs < z1);
             const bool b3 = !b1 && !b2 && (hhnp * hhsp < 0);
             const bool b4 = act_y && !b1 && !b2 && !b3;
             T coef = T(0.5440), hv = h1w, c2 = gg;
@@ -303,13 +347,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             if (b3) {
                 const T sgnm = M::sign_(hhnp - hhsp);
                 coef = -sgnm * T(0.350);
-                hv = (T)(__builtin_fmax(hhn, hhs) - __builtin_fmax(z1, z1w));
+                hv = hv3;
                 c2 = T(2.00) * gg;
                 wet = (h1 > 0) || (h1w > 0);
             }
             T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
-                const T v13 = gg * hn1 * dt2dy * (T)(hhn - h1w - z1w);
+                const T v13 = gg * hn1 * dt2dy * dsurf;
                 const T av = T(0.50) * (uu1_0 + uu1_0w);
                 const T bv = T(0.50) * (uu1_1 + uu1_1w);
                 const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
@@ -341,11 +385,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             const T vnnPe = from_east(vnnP);
             const bool wet = h2 > ewl;
             const T Pc = wet ? P_lake : P_land;
-            const T E = wet ? (T)ev1 : T(0.000);
-            T rr = (T)rr1;
+            const T E = wet ? ev1t : T(0.000);
+            T rr = rr1t;
             if (s.storm_on) {
                 const int gi = g - (r0 - 5);
-                rr = storm1 ? (T)(s.st_peak * (gx1 * gy_s[0][gi < 0 ? 0 : gi])) : T(0);
+                const T gy = gy_s[0][gi < 0 ? 0 : gi];
+                if constexpr (kF32) rr = pk1 * gy;
+                else rr = storm1 ? (T)(s.st_peak * (gx1 * gy)) : T(0);
             }
             T ql = 0;
             for (int i = 0; i < s.nst; ++i)
@@ -381,11 +427,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             const T vnpPe = from_east(vnpP);
             const bool wet = h3 > ewl;
             const T Pc = wet ? P_lake : P_land;
-            const T E = wet ? (T)ev2 : T(0.000);
-            T rr = (T)rr2;
+            const T E = wet ? ev2t : T(0.000);
+            T rr = rr2t;
             if (s.storm_on) {
                 const int qi = q - (r0 - 6);
-                rr = storm2 ? (T)(s.st_peak * (gx2 * gy_s[1][qi < 0 ? 0 : qi])) : T(0);
+                const T gy = gy_s[1][qi < 0 ? 0 : qi];
+                if constexpr (kF32) rr = pk2 * gy;
+                else rr = storm2 ? (T)(s.st_peak * (gx2 * gy)) : T(0);
             }
             T ql = 0;
             for (int i = 0; i < s.nst; ++i)
@@ -409,7 +457,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 vn_out[off] = vnpP;
                 if (a3) {
                     acc_vol += (double)h2v;
-                    acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
+                    if constexpr (kF32) acc_src += (double)(ql + rr * Pc - E);  // x dt2 below
+                    else acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
                     if (hr < 0) acc_clip += -(double)hr;
                 }
                 if (outcell) acc_q += (double)qo_eff * s.dy;
@@ -425,30 +474,31 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         uu1_2 = uu1_1; uu1_1 = uu1_0; uu1_1w = uu1_0w;
         vv1_2 = vv1_1; vv1_2e = vv1_1e; vv1_1 = vv1_0; vv1_1e = vv1_0e; vv1_1w = vv1_0w;
         hm1 = hm0; hn1 = hn0; vva1 = vva0;
+        dzn1 = dzn0; dzw1 = dzw0;
         nc2 = nc1; nc1 = nc0; nc1w = nc0w;
         umnP = umn; vnnP = vnn;
         dryP = dryg; umpP = umpg; vnpP = vnpg;
     }
 
     // ---- per-tile partial sums (fixed-order wave reductions) ----
+    if constexpr (kF32) acc_src *= st.dt2;
     acc_vol = wave_sum(acc_vol);
     acc_src = wave_sum(acc_src);
     acc_clip = wave_sum(acc_clip);
     acc_q = wave_sum(acc_q);
     double cfl = INFINITY;
     if (adapt) {
-        smax_x = wave_max(smax_x);
-        smax_y = wave_max(smax_y);
-        const double wx = smax_x > 0.0 ? s.dx / smax_x : INFINITY;
-        const double wy = smax_y > 0.0 ? s.dy / smax_y : INFINITY;
+        const double mx = wave_max((double)smax_x), my = wave_max((double)smax_y);
+        const double wx = mx > 0.0 ? s.dx / mx : INFINITY;
+        const double wy = my > 0.0 ? s.dy / my : INFINITY;
         cfl = (wx < wy) ? wx : wy;
     }
     if (lane == 0) {
         Partial p;
         p.vol = acc_vol; p.src = acc_src; p.clip = acc_clip; p.qout = acc_q;
-        s.part_out[tile] = p;
+        it.part_out[tile] = p;
         if (adapt && cfl < INFINITY)
-            atomicMin(&s.clk[s.slot_b].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
+            atomicMin(&s.clk[it.slot_b].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
     }
 }
 

@@ -82,6 +82,21 @@ struct StepCommon {
     int nbc; const int *bc_r; const int *bc_c; const int *bc_kind; double *bc_acc;
 };
 
+// The iteration-dependent part of a launch: clock slots, iteration index and
+// the halves of the partial-sum buffer.  Kept apart from StepCommon so that
+// the fused kernel can read StepCommon in place from its descriptor.
+struct Iter {
+    int slot_a, slot_b, slot_c;
+    long long step_index;
+    Partial *part_out;
+    const Partial *part_in;
+};
+
+__host__ __device__ inline Iter iter_of(const StepCommon &s)
+{
+    return Iter{s.slot_a, s.slot_b, s.slot_c, s.step_index, s.part_out, s.part_in};
+}
+
 // Progress of a scenario stepped inside a batch, written by its bookkeeping
 // wave after every iteration it runs.
 struct Status {

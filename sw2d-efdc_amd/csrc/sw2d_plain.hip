@@ -35,7 +35,7 @@ template <bool ADAPT>
 __global__ void __launch_bounds__(256) plain_flux(const PlainArgs a)
 {
     const StepCommon &s = a.s;
-    const StepTime st = step_time(s);
+    const StepTime st = step_time(s, iter_of(s));
     const int R = s.R, C = s.C;
     const double th = s.th, gg = s.g, dx = s.dx, dy = s.dy, dt2 = st.dt2;
     const double *h = a.h, *z = a.z, *hm = a.hm, *hn = a.hn, *uu1 = a.uu1, *vv1 = a.vv1;
@@ -168,7 +168,7 @@ template <int SECOND>
 __global__ void __launch_bounds__(256) plain_continuity(const PlainArgs a)
 {
     const StepCommon &s = a.s;
-    const StepTime st = step_time(s);
+    const StepTime st = step_time(s, iter_of(s));
     const double tf = SECOND ? st.tm : st.t;
     const double dt2 = st.dt2, dx = s.dx, dy = s.dy;
     const int C = s.C;
@@ -307,13 +307,13 @@ __global__ void __launch_bounds__(256) plain_forward(const PlainArgs a)
 __global__ void __launch_bounds__(64) plain_bookkeeping(const PlainArgs a)
 {
     const StepCommon &s = a.s;
-    const StepTime st = step_time(s);
+    const StepTime st = step_time(s, iter_of(s));
     const int C = s.C;
     auto vel = [&](int r, int j, double &hh, double &ua, double &va) {
         const int p = r * C + j;
         hh = a.h[p]; ua = a.uua[p]; va = a.vva[p];
     };
-    bookkeeping(s, st, vel, (int)blockIdx.x, threadIdx.x);
+    bookkeeping(s, iter_of(s), st, vel, (int)blockIdx.x, threadIdx.x);
 }
 
 // init_inf + initiald (SW2D-EFDC.cu:29-91) and the initial CFL minimum

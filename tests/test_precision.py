@@ -1,0 +1,75 @@
+"""Precision form on the pond-break case: FP32 state/fluxes with FP64 bed,
+time step, volume ledger and hydrograph, against the CPU FP64 oracle.
+
+The case is the 4096^2 pond break at a size the oracle finishes in seconds:
+flat floodplain + fBm 0.5 m, dx = 2 m, a square reservoir held 4 m deep and
+released at t = 0 onto a dry bed, n = 0.035, no rain, CFL-adaptive step.
+Reported and bounded: max|dh| <= 1 mm, L2(h), the first-arrival time of the
+front (h > 1 cm, sampled every 5 s) and the relative volume error of the FP32
+ledger (<= 1e-6).  The FP64 form is checked value for value on the same run.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def pond(cases, n=384, res=64):
+    spec = cases.config2(rows=n, cols=n, t_end=240.0)
+    # the reservoir of config 2 is 200 x 200 cells at 4096^2; scale it
+    z = spec.z
+    r0 = c0 = n // 8
+    mask = np.zeros_like(z, dtype=bool)
+    mask[r0:r0 + res, c0:c0 + res] = True
+    level = z[mask].mean() + 4.0
+    h0 = np.where(mask, np.maximum(level - z, 0.0), 0.0)
+    return spec, z, h0
+
+
+def run(gpu_or_oracle, t_end, sample=5.0):
+    """Step to t_end, sampling h every `sample` seconds for arrival times."""
+    arrival = None
+    t = 0.0
+    while t < t_end - 1e-9:
+        t = min(t + sample, t_end)
+        gpu_or_oracle.run_until(t)
+        h = gpu_or_oracle.field("h")
+        tt = gpu_or_oracle.time
+        if arrival is None:
+            arrival = np.full(h.shape, np.inf)
+        arrival = np.where((h > 0.01) & np.isinf(arrival), tt, arrival)
+    return arrival
+
+
+def test_pond_break_fp32_form(pkg, po, cases):
+    spec, z, h0 = pond(cases)
+    kw = dict(dt=spec.dt, manning=spec.manning, adaptive=True, courant=spec.courant)
+    o = po.Oracle(z, h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+    o.set_adaptive(spec.courant, 2.0 * spec.dt)
+    g64 = pkg.Case(z, h0, spec.cellsize, precision="fp64", **kw)
+    g32 = pkg.Case(z, h0, spec.cellsize, precision="fp32", **kw)
+    a_o = run(o, spec.t_end)
+    a_64 = run(g64, spec.t_end)
+    a_32 = run(g32, spec.t_end)
+    h_o = o.field("h")
+    # FP64 form: value for value
+    assert np.array_equal(g64.field("h"), h_o)
+    assert np.array_equal(a_64, a_o)
+    # FP32 form
+    dh = g32.field("h") - h_o
+    max_dh = float(np.max(np.abs(dh)))
+    l2 = float(np.sqrt(np.mean(dh * dh)))
+    both = np.isfinite(a_o) & np.isfinite(a_32)
+    arr_err = float(np.max(np.abs(a_32[both] - a_o[both]))) if both.any() else 0.0
+    reached_o, reached_32 = int(np.isfinite(a_o).sum()), int(np.isfinite(a_32).sum())
+    b = g32.balance()
+    vol_err = abs(b["error"]) / b["volume"]
+    print(f"pond break {z.shape}: max|dh| {max_dh:.3e} m, L2 {l2:.3e} m, arrival error "
+          f"{arr_err:.1f} s (cells reached: oracle {reached_o}, fp32 {reached_32}), "
+          f"volume error {vol_err:.2e}")
+    assert max_dh <= 1e-3
+    assert vol_err <= 1e-6
+    assert abs(g32.time - o.time) <= 1e-6 * spec.t_end
+    # the front reaches the same cells, within one sampling interval
+    assert abs(reached_32 - reached_o) <= 0.01 * reached_o
+    assert arr_err <= 5.0
