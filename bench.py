@@ -53,9 +53,11 @@ def log(msg):
 def bytes_per_cell(precision, has_class_map):
     """Compulsory HBM bytes of one cell-update of the fused kernel: read h, um,
     vn and the FP64 bed z, write h, um, vn (+1 byte Manning class when a class
-    map is used): 57 B in FP64, 33 B in the FP32 form."""
-    e = 8 if precision == "fp64" else 4
-    return 6 * e + 8 + (1 if has_class_map else 0)
+    map is used); the FP32 form also reads and writes the low word of its
+    compensated depth: 57 B in FP64, 41 B in the FP32 form."""
+    if precision == "fp64":
+        return 6 * 8 + 8 + (1 if has_class_map else 0)
+    return 6 * 4 + 8 + 2 * 4 + (1 if has_class_map else 0)
 
 
 def make_case(pkg, spec, precision, kernel, tile_rows):
@@ -178,7 +180,8 @@ def batch_leg(pkg, cases, n, plain_rates):
         "unit": "cell-updates/s",
         "scenarios_per_s": n / wall,
         "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell_update": 32,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "bytes_per_cell_update": bytes_per_cell("fp32", False),
                      "kernel_ms_total": kms, "launches": launches},
         "max_volume_error_rel": vol_err,
         "mean_q_peak_m3s": float(np.mean(q_peak)),
@@ -242,7 +245,7 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates):
         "pool_peak_bytes": st["pool_peak"],
         "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
-                     "bytes_per_cell_update": 32},
+                     "bytes_per_cell_update": bytes_per_cell("fp32", False)},
         "max_volume_error_rel": vol_err,
         "offered_cell_updates_per_s": float(sum(s.cells * k for s, k in zip(specs, iters))
                                             / max(arrivals[-1], 1e-9)),

@@ -38,8 +38,8 @@
 #endif
 
 /* ------------------------------------------------------------------------
- * Project math: pow(x, 1.3333330) and exp(x) from IEEE-exact operations only
- * (+ - * /, floor, exponent bit manipulation).  The HIP library evaluates
+ * Project math: pow(x, 1.3333330) and exp(x) from correctly rounded operations
+ * only (+ - * /, fma, floor, exponent bit manipulation).  The HIP library evaluates
  * the identical sequence, so both sides round identically.
  *
  * Method: argument reduction x = 2^e * m, m in (sqrt(1/2), sqrt(2)];
@@ -68,9 +68,20 @@ static double orc_log_pos(double x)
     double f = m - 1.0;
     double s = f / (2.0 + f);
     double z = s * s;
-    double R = z * ((2.0 / 3.0) + z * ((2.0 / 5.0) + z * ((2.0 / 7.0) + z * ((2.0 / 9.0) +
-               z * ((2.0 / 11.0) + z * ((2.0 / 13.0) + z * ((2.0 / 15.0) + z * ((2.0 / 17.0) +
-               z * ((2.0 / 19.0) + z * ((2.0 / 21.0) + z * ((2.0 / 23.0) + z * (2.0 / 25.0))))))))))));
+    /* R(z) = sum_{k=1..12} 2 z^k / (2k+1), Horner with correctly rounded fma */
+    double R = 2.0 / 25.0;
+    R = fma(z, R, 2.0 / 23.0);
+    R = fma(z, R, 2.0 / 21.0);
+    R = fma(z, R, 2.0 / 19.0);
+    R = fma(z, R, 2.0 / 17.0);
+    R = fma(z, R, 2.0 / 15.0);
+    R = fma(z, R, 2.0 / 13.0);
+    R = fma(z, R, 2.0 / 11.0);
+    R = fma(z, R, 2.0 / 9.0);
+    R = fma(z, R, 2.0 / 7.0);
+    R = fma(z, R, 2.0 / 5.0);
+    R = fma(z, R, 2.0 / 3.0);
+    R = z * R;
     double hf = 0.5 * f * f;
     double lm = f - (hf - s * (hf + R));
     double de = (double)e;
@@ -85,19 +96,20 @@ double orc_exp(double t)
     double kd = floor(t * ORC_INV_LN2 + 0.5);
     int k = (int)kd;
     double r = (t - kd * ORC_LN2_HI) - kd * ORC_LN2_LO;
-    double p = 1.0 / 6227020800.0;           /* 1/13! */
-    p = 1.0 / 479001600.0 + r * p;           /* 1/12! */
-    p = 1.0 / 39916800.0 + r * p;            /* 1/11! */
-    p = 1.0 / 3628800.0 + r * p;             /* 1/10! */
-    p = 1.0 / 362880.0 + r * p;              /* 1/9!  */
-    p = 1.0 / 40320.0 + r * p;               /* 1/8!  */
-    p = 1.0 / 5040.0 + r * p;                /* 1/7!  */
-    p = 1.0 / 720.0 + r * p;                 /* 1/6!  */
-    p = 1.0 / 120.0 + r * p;                 /* 1/5!  */
-    p = 1.0 / 24.0 + r * p;                  /* 1/4!  */
-    p = 1.0 / 6.0 + r * p;                   /* 1/3!  */
-    p = 0.5 + r * p;                         /* 1/2!  */
-    double er = 1.0 + r * (1.0 + r * p);
+    /* Taylor to r^13, Horner with correctly rounded fma */
+    double p = 1.0 / 6227020800.0;       /* 1/13! */
+    p = fma(r, p, 1.0 / 479001600.0);    /* 1/12! */
+    p = fma(r, p, 1.0 / 39916800.0);     /* 1/11! */
+    p = fma(r, p, 1.0 / 3628800.0);      /* 1/10! */
+    p = fma(r, p, 1.0 / 362880.0);       /* 1/9!  */
+    p = fma(r, p, 1.0 / 40320.0);        /* 1/8!  */
+    p = fma(r, p, 1.0 / 5040.0);         /* 1/7!  */
+    p = fma(r, p, 1.0 / 720.0);          /* 1/6!  */
+    p = fma(r, p, 1.0 / 120.0);          /* 1/5!  */
+    p = fma(r, p, 1.0 / 24.0);           /* 1/4!  */
+    p = fma(r, p, 1.0 / 6.0);            /* 1/3!  */
+    p = fma(r, p, 0.5);                  /* 1/2!  */
+    double er = fma(r, fma(r, p, 1.0), 1.0);
     /* 2^k split in two factors so that k down to -1022-52 stays exact */
     int k1 = k / 2, k2 = k - k1;
     double s1 = orc_bits2d((uint64_t)(k1 + 1023) << 52);

@@ -97,6 +97,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const gptr<T> h_out = (gptr<T>)d.st[ib ^ 1][0];
     const gptr<T> um_out = (gptr<T>)d.st[ib ^ 1][1];
     const gptr<T> vn_out = (gptr<T>)d.st[ib ^ 1][2];
+    const gptr<const float> lo_in = (gptr<const float>)d.st[ib][3];
+    const gptr<float> lo_out = (gptr<float>)d.st[ib ^ 1][3];
     const gptr<const double> zg = (gptr<const double>)d.z;
     const gptr<const uint8_t> clsg = (gptr<const uint8_t>)d.cls;
 
@@ -194,7 +196,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // values (0, inactive), so the loads need no branch.
     const int kbeg = r0 - 3, kend = r1 + 3;
     const int jc = colin ? j : (j < 0 ? 0 : C - 1);
-    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
+    // FP32 form: also the depth's low word of row k-3 (where the second
+    // continuity of the same iteration stores row k-3)
+    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc, float &lo) {
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
         const int off = kc * P + jc;
@@ -205,17 +209,27 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         vv = in ? lv : T(0);
         zz = in ? lz : __builtin_nan("");
         cc = (s.ncls_on && in) ? clsg[off] : (uint8_t)0;
+        if constexpr (kF32) {
+            const int kq = k - 3;
+            const int kqc = kq < 0 ? 0 : (kq >= R ? R - 1 : kq);
+            const float ll = lo_in[kqc * P + jc];
+            lo = (colin && kq >= 0 && kq < R) ? ll : 0.0f;
+        } else {
+            lo = 0.0f;
+        }
     };
     T nh, nu, nv;
     double nz;
     uint8_t ncl;
-    load_row(kbeg, nh, nu, nv, nz, ncl);
+    float nlo;
+    load_row(kbeg, nh, nu, nv, nz, ncl, nlo);
 
     for (int k = kbeg; k < kend; ++k) {
         const T h0 = nh, u0 = nu, v0 = nv;
         const double z0 = nz;
         const uint8_t cl0 = ncl;
-        load_row(k + 1, nh, nu, nv, nz, ncl);
+        const float lo3 = nlo;  // low word of row k-3 (FP32 form)
+        load_row(k + 1, nh, nu, nv, nz, ncl, nlo);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
         const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
 
@@ -261,8 +275,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // rows before r0 - 1 feed no output row: the warm-up only builds the window
         const bool flux_um = k >= r0, flux_vn = k >= r0 && k <= r1 + 1;
         // x direction: um at the north face of (f, j); w = row f-1, e = row f.
-        // Dry-bed branches B1-B3 all read (coef * hv) * sqrt(c2 * hv); B4 (the
-        // momentum update) runs only when a lane of the wave needs it.
+        // Dry-bed branches B1-B3 all read (coef * hv) * sqrt(c2 * hv); each of
+        // that and B4 (the momentum update) runs only when a lane needs it.
         const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th));
         if (__any(act_x)) {
             // water surfaces: FP64 (the reference's arithmetic) or, in the
@@ -295,7 +309,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 c2 = T(2.00) * gg;
                 wet = (h1 > 0) || (h2 > 0);
             }
-            T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
+            T val = 0;
+            if (__any(act_x && !b4)) val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
                 const T u13 = gg * hm1 * dt2dx * dsurf;
                 const T ur = T(0.50) * (uu1_0 + uu1_1);
@@ -351,7 +366,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 c2 = T(2.00) * gg;
                 wet = (h1 > 0) || (h1w > 0);
             }
-            T val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
+            T val = 0;
+            if (__any(act_y && !b4)) val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
                 const T v13 = gg * hn1 * dt2dy * dsurf;
                 const T av = T(0.50) * (uu1_0 + uu1_0w);
@@ -446,20 +462,43 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 else if (s.out_edge == 3) vnw = -qo_eff;
                 else vne = qo_eff;
             }
-            const T hr = h3 - dt2 * (M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) -
-                                     ql - rr * Pc + E);
-            T h2v = M::max_(hr, T(0.00));
-            if (!a3) h2v = 0.00;
+            const T rhs = M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) - ql -
+                          rr * Pc + E;
+            T h2v, lo_new = 0;
+            double clip = 0.0;
+            if constexpr (kF32) {
+                // compensated depth: (h3 + lo3) - dt2 rhs, the rounding error
+                // of the sum kept exactly in the low word (TwoSum), so that
+                // FP32 rounding does not drain or add volume coherently
+                const T tt = lo3 - dt2 * rhs;
+                const T sm = h3 + tt;
+                const T bp = sm - h3;
+                const T er = (h3 - (sm - bp)) + (tt - bp);
+                const double hrd = (double)sm + (double)er;
+                h2v = sm;
+                lo_new = er;
+                if (hrd < 0.0) { clip = -hrd; h2v = 0; lo_new = 0; }
+            } else {
+                const T hr = h3 - dt2 * rhs;
+                h2v = M::max_(hr, T(0.00));
+                if (hr < 0) clip = -(double)hr;
+            }
+            if (!a3) { h2v = 0.00; lo_new = 0; }
             if (colout && q >= r0 && q < r1) {
                 const long off = (long)q * P + j;
                 h_out[off] = h2v;
                 um_out[off] = umpP;
                 vn_out[off] = vnpP;
+                if constexpr (kF32) lo_out[off] = (float)lo_new;
                 if (a3) {
-                    acc_vol += (double)h2v;
-                    if constexpr (kF32) acc_src += (double)(ql + rr * Pc - E);  // x dt2 below
-                    else acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
-                    if (hr < 0) acc_clip += -(double)hr;
+                    if constexpr (kF32) {
+                        acc_vol += (double)h2v + (double)lo_new;
+                        acc_src += (double)(ql + rr * Pc - E);  // x dt2 below
+                    } else {
+                        acc_vol += (double)h2v;
+                        acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
+                    }
+                    acc_clip += clip;
                 }
                 if (outcell) acc_q += (double)qo_eff * s.dy;
             }
@@ -543,8 +582,9 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
 // one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva)
 template <typename T>
 __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
-                                                    const double *z, int R, int C, int P,
-                                                    double th, int field, double *out)
+                                                    const float *lo, const double *z, int R,
+                                                    int C, int P, double th, int field,
+                                                    double *out)
 {
     PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
     const long n = (long)R * C;
@@ -554,7 +594,8 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
         const long o = (long)r * P + j;
         double v = 0;
         switch (field) {
-        case SW2D_FIELD_H: v = h[o]; break;
+        // the FP32 form's depth is its compensated value (high + low word)
+        case SW2D_FIELD_H: v = lo ? (double)h[o] + (double)lo[o] : (double)h[o]; break;
         case SW2D_FIELD_UM: v = um[o]; break;
         case SW2D_FIELD_VN: v = vn[o]; break;
         case SW2D_FIELD_HM: v = (r == 0) ? h[o] : T(0.50) * (h[o] + h[o - P]); break;
@@ -602,7 +643,8 @@ static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev)
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(derive_field<T>, dim3(blocks), dim3(256), 0, h->stream,
                        (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
-                       (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev);
+                       (const float *)h->hlo[h->cur], (const double *)h->z, h->R, h->C, h->P,
+                       h->prm.th, field, out_dev);
     return hipGetLastError();
 }
 

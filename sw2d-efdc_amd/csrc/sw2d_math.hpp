@@ -1,18 +1,19 @@
 // sw2d_math.hpp -- device math of the SW2D kernels.
 //
 // FP64: pow(x, 1.3333330) (the Manning depth exponent of SW2D-EFDC.cu:194,
-// 311) and exp() are built from IEEE-exact operations only (+ - * /, floor,
-// exponent bit manipulation), so that the result is a fixed function of the
-// input on every machine: the CPU FP64 check of this library evaluates the
-// same sequence and must agree bit for bit.  Translation units that include
-// this header are compiled with -ffp-contract=off.
+// 311) and exp() are built from correctly rounded operations only (+ - * /,
+// explicit fma, floor, exponent bit manipulation), so that the result is a
+// fixed function of the input on every machine: the CPU FP64 check of this
+// library evaluates the same sequence and must agree bit for bit.
+// Translation units that include this header are compiled with
+// -ffp-contract=off (no fusing beyond the fma written here).
 //
 // Method: x = 2^e m, m in (sqrt(1/2), sqrt(2)]; log m = 2 atanh(s),
 // s = (m-1)/(m+1), arranged as f - (hf - s (hf + R)), hf = f^2/2,
 // R(z) = sum_k 2 z^k / (2k+1), z = s^2 (the atanh-series arrangement of Sun's
 // fdlibm e_log.c with plain Taylor coefficients); exp(t) = 2^k exp(r),
 // r = t - k ln2 with a two-constant (Cody-Waite) ln2, exp(r) by Taylor to
-// r^13.  Error against a correctly rounded pow: under 32 ulp for x in
+// r^13; both polynomials by Horner's rule with fma.  Error against a correctly rounded pow: under 32 ulp for x in
 // [1e-5, 1e3].
 //
 // FP32: the hardware log2/exp2 approximations (about 1 ulp each).
@@ -37,9 +38,19 @@ __device__ __forceinline__ double log_pos(double x)
     double f = m - 1.0;
     double s = f / (2.0 + f);
     double z = s * s;
-    double R = z * ((2.0 / 3.0) + z * ((2.0 / 5.0) + z * ((2.0 / 7.0) + z * ((2.0 / 9.0) +
-               z * ((2.0 / 11.0) + z * ((2.0 / 13.0) + z * ((2.0 / 15.0) + z * ((2.0 / 17.0) +
-               z * ((2.0 / 19.0) + z * ((2.0 / 21.0) + z * ((2.0 / 23.0) + z * (2.0 / 25.0))))))))))));
+    double R = 2.0 / 25.0;
+    R = __builtin_fma(z, R, 2.0 / 23.0);
+    R = __builtin_fma(z, R, 2.0 / 21.0);
+    R = __builtin_fma(z, R, 2.0 / 19.0);
+    R = __builtin_fma(z, R, 2.0 / 17.0);
+    R = __builtin_fma(z, R, 2.0 / 15.0);
+    R = __builtin_fma(z, R, 2.0 / 13.0);
+    R = __builtin_fma(z, R, 2.0 / 11.0);
+    R = __builtin_fma(z, R, 2.0 / 9.0);
+    R = __builtin_fma(z, R, 2.0 / 7.0);
+    R = __builtin_fma(z, R, 2.0 / 5.0);
+    R = __builtin_fma(z, R, 2.0 / 3.0);
+    R = z * R;
     double hf = 0.5 * f * f;
     double lm = f - (hf - s * (hf + R));
     double de = (double)e;
@@ -55,18 +66,18 @@ __device__ __forceinline__ double exp_d(double t)
     int k = (int)kd;
     double r = (t - kd * kLn2Hi) - kd * kLn2Lo;
     double p = 1.0 / 6227020800.0;
-    p = 1.0 / 479001600.0 + r * p;
-    p = 1.0 / 39916800.0 + r * p;
-    p = 1.0 / 3628800.0 + r * p;
-    p = 1.0 / 362880.0 + r * p;
-    p = 1.0 / 40320.0 + r * p;
-    p = 1.0 / 5040.0 + r * p;
-    p = 1.0 / 720.0 + r * p;
-    p = 1.0 / 120.0 + r * p;
-    p = 1.0 / 24.0 + r * p;
-    p = 1.0 / 6.0 + r * p;
-    p = 0.5 + r * p;
-    double er = 1.0 + r * (1.0 + r * p);
+    p = __builtin_fma(r, p, 1.0 / 479001600.0);
+    p = __builtin_fma(r, p, 1.0 / 39916800.0);
+    p = __builtin_fma(r, p, 1.0 / 3628800.0);
+    p = __builtin_fma(r, p, 1.0 / 362880.0);
+    p = __builtin_fma(r, p, 1.0 / 40320.0);
+    p = __builtin_fma(r, p, 1.0 / 5040.0);
+    p = __builtin_fma(r, p, 1.0 / 720.0);
+    p = __builtin_fma(r, p, 1.0 / 120.0);
+    p = __builtin_fma(r, p, 1.0 / 24.0);
+    p = __builtin_fma(r, p, 1.0 / 6.0);
+    p = __builtin_fma(r, p, 0.5);
+    double er = __builtin_fma(r, __builtin_fma(r, p, 1.0), 1.0);
     int k1 = k / 2, k2 = k - k1;
     double s1 = __longlong_as_double((long long)((uint64_t)(k1 + 1023) << 52));
     double s2 = __longlong_as_double((long long)((uint64_t)(k2 + 1023) << 52));
