@@ -53,11 +53,9 @@ def log(msg):
 def bytes_per_cell(precision, has_class_map):
     """Compulsory HBM bytes of one cell-update of the fused kernel: read h, um,
     vn and the FP64 bed z, write h, um, vn (+1 byte Manning class when a class
-    map is used); the FP32 form also reads and writes the low word of its
-    compensated depth: 57 B in FP64, 41 B in the FP32 form."""
-    if precision == "fp64":
-        return 6 * 8 + 8 + (1 if has_class_map else 0)
-    return 6 * 4 + 8 + 2 * 4 + (1 if has_class_map else 0)
+    map is used): 57 B in FP64, 33 B in the FP32 form."""
+    e = 8 if precision == "fp64" else 4
+    return 6 * e + 8 + (1 if has_class_map else 0)
 
 
 def make_case(pkg, spec, precision, kernel, tile_rows):

@@ -68,7 +68,6 @@ Scenario make_scenario(sw2d_handle *h)
         d.st[b][0] = h->h[b];
         d.st[b][1] = h->um[b];
         d.st[b][2] = h->vn[b];
-        d.st[b][3] = h->hlo[b];
     }
     d.z = (const double *)h->z;
     d.cls = h->cls;
@@ -294,9 +293,6 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
             if (h->esize == 8) { if ((rc = dalloc(h, (double **)b, n))) return bail(rc); }
             else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
         }
-        if (h->esize == 4)
-            for (int b = 0; b < 2; b++)
-                if ((rc = dalloc(h, &h->hlo[b], n))) return bail(rc);
         if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
         if ((rc = dalloc(h, &h->dscen, 1))) return bail(rc);
         // pitched images: inactive cells carry NaN in z, which stays FP64
@@ -897,7 +893,7 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
     if (check_idle(h) || !bytes) return SW2D_E_ARG;
     long long b = 0;
     const size_t RC = (size_t)h->R * h->C;
-    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? (long long)h->R * h->P : 0) + (h->hlo[0] ? (long long)2 * h->R * h->P * 4 : 0);
+    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? (long long)h->R * h->P : 0);
     else b += (long long)16 * h->NA * 8;
     b += (long long)2 * h->nparts * sizeof(Partial);
     *bytes = b;

@@ -222,7 +222,6 @@ size_t sw2d_batch_footprint(const sw2d_grid_desc *grid, const sw2d_params *param
     const size_t ntiles = ((C + 57) / 58) * ((R + H - 1) / H);
     size_t b = 0;
     b += 6 * pool_bytes_of(R * P * es);                         // h, um, vn ping-pong
-    if (es == 4) b += 2 * pool_bytes_of(R * P * 4);              // low words of the FP32 depth
     b += pool_bytes_of(R * P * 8);                               // z
     b += pool_bytes_of(sizeof(Scenario));                        // descriptor
     b += pool_bytes_of(3 * sizeof(Clock));

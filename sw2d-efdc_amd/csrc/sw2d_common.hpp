@@ -53,6 +53,19 @@ __device__ __forceinline__ double source_rate(const StepCommon &s, int i, double
     return ql / (s.dx * s.dy);
 }
 
+// uniform number in [0, 1) from (cell, iteration): a 32-bit integer hash
+// (multiply-xorshift finaliser), 24 bits of it scaled by 2^-24
+__device__ __forceinline__ float cell_uniform(uint32_t cell, uint32_t iter)
+{
+    uint32_t x = cell * 0x9E3779B1u ^ iter * 0x85EBCA77u;
+    x ^= x >> 16;
+    x *= 0x7FEB352Du;
+    x ^= x >> 15;
+    x *= 0x846CA68Bu;
+    x ^= x >> 16;
+    return (float)(x >> 8) * 5.9604644775390625e-08f;
+}
+
 // one factor of the moving Gaussian storm
 __device__ __forceinline__ double storm_factor(double d, double sigma)
 {

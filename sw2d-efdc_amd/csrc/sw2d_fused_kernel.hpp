@@ -97,8 +97,6 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const gptr<T> h_out = (gptr<T>)d.st[ib ^ 1][0];
     const gptr<T> um_out = (gptr<T>)d.st[ib ^ 1][1];
     const gptr<T> vn_out = (gptr<T>)d.st[ib ^ 1][2];
-    const gptr<const float> lo_in = (gptr<const float>)d.st[ib][3];
-    const gptr<float> lo_out = (gptr<float>)d.st[ib ^ 1][3];
     const gptr<const double> zg = (gptr<const double>)d.z;
     const gptr<const uint8_t> clsg = (gptr<const uint8_t>)d.cls;
 
@@ -196,9 +194,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // values (0, inactive), so the loads need no branch.
     const int kbeg = r0 - 3, kend = r1 + 3;
     const int jc = colin ? j : (j < 0 ? 0 : C - 1);
-    // FP32 form: also the depth's low word of row k-3 (where the second
-    // continuity of the same iteration stores row k-3)
-    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc, float &lo) {
+    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
         const int off = kc * P + jc;
@@ -209,27 +205,17 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         vv = in ? lv : T(0);
         zz = in ? lz : __builtin_nan("");
         cc = (s.ncls_on && in) ? clsg[off] : (uint8_t)0;
-        if constexpr (kF32) {
-            const int kq = k - 3;
-            const int kqc = kq < 0 ? 0 : (kq >= R ? R - 1 : kq);
-            const float ll = lo_in[kqc * P + jc];
-            lo = (colin && kq >= 0 && kq < R) ? ll : 0.0f;
-        } else {
-            lo = 0.0f;
-        }
     };
     T nh, nu, nv;
     double nz;
     uint8_t ncl;
-    float nlo;
-    load_row(kbeg, nh, nu, nv, nz, ncl, nlo);
+    load_row(kbeg, nh, nu, nv, nz, ncl);
 
     for (int k = kbeg; k < kend; ++k) {
         const T h0 = nh, u0 = nu, v0 = nv;
         const double z0 = nz;
         const uint8_t cl0 = ncl;
-        const float lo3 = nlo;  // low word of row k-3 (FP32 form)
-        load_row(k + 1, nh, nu, nv, nz, ncl, nlo);
+        load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
         const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
 
@@ -464,40 +450,46 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             }
             const T rhs = M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) - ql -
                           rr * Pc + E;
-            T h2v, lo_new = 0;
+            T h2v;
             double clip = 0.0;
             if constexpr (kF32) {
-                // compensated depth: (h3 + lo3) - dt2 rhs, the rounding error
-                // of the sum kept exactly in the low word (TwoSum), so that
-                // FP32 rounding does not drain or add volume coherently
-                const T tt = lo3 - dt2 * rhs;
+                // FP32 form: the depth update h3 - dt2 rhs is rounded
+                // stochastically.  TwoSum gives the exact rounding error er of
+                // sm = fl(h3 + tt); the result moves to the FP32 neighbour of sm
+                // on er's side with probability |er| / spacing, so rounding
+                // errors average out instead of adding up coherently (every
+                // cell of a uniformly wetted slope rounds the same small rain
+                // increment the same way).  The random number is a fixed hash
+                // of (cell, iteration): results are reproducible and do not
+                // depend on the tiling or on other scenarios.
+                const T tt = -dt2 * rhs;
                 const T sm = h3 + tt;
                 const T bp = sm - h3;
                 const T er = (h3 - (sm - bp)) + (tt - bp);
-                const double hrd = (double)sm + (double)er;
                 h2v = sm;
-                lo_new = er;
-                if (hrd < 0.0) { clip = -hrd; h2v = 0; lo_new = 0; }
+                if (sm > T(0) && er != T(0)) {
+                    const uint32_t bits = __float_as_uint(sm);
+                    const T nb = __uint_as_float(er > T(0) ? bits + 1u : bits - 1u);
+                    const T pr = er * __builtin_amdgcn_rcpf(nb - sm);
+                    if (cell_uniform((uint32_t)(q * C + j), (uint32_t)it.step_index) < pr) h2v = nb;
+                }
+                const double hrd = (double)sm + (double)er;
+                if (hrd < 0.0) { clip = -hrd; h2v = 0; }
             } else {
                 const T hr = h3 - dt2 * rhs;
                 h2v = M::max_(hr, T(0.00));
                 if (hr < 0) clip = -(double)hr;
             }
-            if (!a3) { h2v = 0.00; lo_new = 0; }
+            if (!a3) h2v = 0.00;
             if (colout && q >= r0 && q < r1) {
                 const long off = (long)q * P + j;
                 h_out[off] = h2v;
                 um_out[off] = umpP;
                 vn_out[off] = vnpP;
-                if constexpr (kF32) lo_out[off] = (float)lo_new;
                 if (a3) {
-                    if constexpr (kF32) {
-                        acc_vol += (double)h2v + (double)lo_new;
-                        acc_src += (double)(ql + rr * Pc - E);  // x dt2 below
-                    } else {
-                        acc_vol += (double)h2v;
-                        acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
-                    }
+                    acc_vol += (double)h2v;
+                    if constexpr (kF32) acc_src += (double)(ql + rr * Pc - E);  // x dt2 below
+                    else acc_src += st.dt2 * ((double)ql + (double)rr * (double)Pc - (double)E);
                     acc_clip += clip;
                 }
                 if (outcell) acc_q += (double)qo_eff * s.dy;
@@ -582,9 +574,8 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
 // one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva)
 template <typename T>
 __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
-                                                    const float *lo, const double *z, int R,
-                                                    int C, int P, double th, int field,
-                                                    double *out)
+                                                    const double *z, int R, int C, int P,
+                                                    double th, int field, double *out)
 {
     PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
     const long n = (long)R * C;
@@ -594,8 +585,7 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
         const long o = (long)r * P + j;
         double v = 0;
         switch (field) {
-        // the FP32 form's depth is its compensated value (high + low word)
-        case SW2D_FIELD_H: v = lo ? (double)h[o] + (double)lo[o] : (double)h[o]; break;
+        case SW2D_FIELD_H: v = h[o]; break;
         case SW2D_FIELD_UM: v = um[o]; break;
         case SW2D_FIELD_VN: v = vn[o]; break;
         case SW2D_FIELD_HM: v = (r == 0) ? h[o] : T(0.50) * (h[o] + h[o - P]); break;
@@ -643,8 +633,7 @@ static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev)
     if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(derive_field<T>, dim3(blocks), dim3(256), 0, h->stream,
                        (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
-                       (const float *)h->hlo[h->cur], (const double *)h->z, h->R, h->C, h->P,
-                       h->prm.th, field, out_dev);
+                       (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev);
     return hipGetLastError();
 }
 

@@ -114,8 +114,7 @@ struct Status {
 // fixed sequence of six launches can be captured once and replayed.
 struct Scenario {
     StepCommon s;            // slot_*, step_index and part_in/out are filled per launch
-    void *st[2][4];          // (h, um, vn, h_lo) of ping-pong buffer 0 and 1; iteration k
-                             // reads k%2; h_lo (FP32 form only) is the low word of the depth
+    void *st[2][3];          // (h, um, vn) of ping-pong buffer 0 and 1; iteration k reads k%2
     const double *z;         // bed elevation (FP64 in both forms); NaN marks an inactive cell
     const uint8_t *cls;      // Manning class per cell (ncls_on)
     int P;                   // row pitch in elements
@@ -149,7 +148,6 @@ struct sw2d_handle {
     int esize = 8;                      // bytes per state element
     // fused-path state (ping-pong)
     void *h[2] = {nullptr, nullptr}, *um[2] = {nullptr, nullptr}, *vn[2] = {nullptr, nullptr};
-    float *hlo[2] = {nullptr, nullptr};  // FP32 form: low word of the compensated depth
     void *z = nullptr;
     uint8_t *cls = nullptr;
     int cur = 0;

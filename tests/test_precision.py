@@ -77,8 +77,9 @@ def test_pond_break_fp32_form(pkg, po, cases):
 
 def test_fp32_volume_ledger_rain_ensemble(pkg, cases):
     """Rain onto dry beds is where FP32 depths round coherently (every cell
-    adds the same small increment); the compensated depth keeps the FP32
-    form's volume balance within 1e-6 of the FP64 ledger over 2 h storms."""
+    adds the same small increment); stochastic rounding of the depth update
+    keeps the FP32 form's volume balance within 1e-6 of the FP64 ledger over
+    2 h storms."""
     batch = pkg.Batch(precision="fp32", pool_bytes=512 << 20)
     members = []
     for k in range(6):
