@@ -29,6 +29,7 @@ import numpy as np
 
 ROOT = pathlib.Path(__file__).resolve().parent
 METRIC = "cell-updates/s + % HBM roofline; p50/p99 scenario latency vs plain FP64 GPU"
+TRAFFIC_TABLE = ROOT / "profiles" / "fused_step_2048_dropin_traffic.json"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -74,6 +75,19 @@ def make_case(pkg, spec, precision, kernel, tile_rows):
     if spec.lake_mask is not None:
         c.set_lake_bc(spec.lake_mask)
     return c
+
+
+def measured_traffic(precision, tile_rows, grid):
+    """HBM bytes per launch of the fused kernel at this tile height, from the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE table committed under profiles/ (None
+    when the table does not cover this case)."""
+    try:
+        t = json.load(open(TRAFFIC_TABLE))
+    except (OSError, ValueError):
+        return None
+    if list(t.get("grid", [])) != list(grid):
+        return None
+    return t.get("bytes_per_launch", {}).get(precision, {}).get(str(tile_rows))
 
 
 def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps):
@@ -252,6 +266,48 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates):
     }
 
 
+def pond_break_leg(pkg, cases, side, sample=30.0):
+    """The pond-break case (square reservoir 4 m deep released onto a dry
+    fBm floodplain, dx 2 m, n 0.035, adaptive step, 30 min) in the FP32 form
+    beside the FP64 form, which equals the CPU FP64 oracle value for value:
+    max|dh|, L2(h), first-arrival time of h > 1 cm (sampled every `sample` s)
+    and the FP32 ledger's volume error."""
+    spec = cases.config2(rows=side, cols=side)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        c = pkg.Case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning,
+                     precision=prec, adaptive=True, courant=spec.courant)
+        arrival = np.full(spec.z.shape, np.inf)
+        t, iters, t0 = 0.0, 0, time.perf_counter()
+        while t < spec.t_end - 1e-9:
+            t = min(t + sample, spec.t_end)
+            iters += c.run_until(t)
+            h = c.field("h")
+            arrival = np.where((h > 0.01) & np.isinf(arrival), c.time, arrival)
+        b = c.balance()
+        res[prec] = dict(h=h, arrival=arrival, time=c.time, iters=iters,
+                         wall=time.perf_counter() - t0, vol_err=b["error"] / b["volume"])
+        c.close()
+    dh = res["fp32"]["h"] - res["fp64"]["h"]
+    a64, a32 = res["fp64"]["arrival"], res["fp32"]["arrival"]
+    both = np.isfinite(a64) & np.isfinite(a32)
+    return {
+        "workload": f"pond break {side}x{side}, dx 2 m, 400 m square reservoir at 4 m, n 0.035, "
+                    f"adaptive step, {spec.t_end:.0f} s",
+        "max_abs_dh_m": float(np.max(np.abs(dh))),
+        "l2_dh_m": float(np.sqrt(np.mean(dh * dh))),
+        "max_arrival_error_s": float(np.max(np.abs(a32[both] - a64[both]))) if both.any() else 0.0,
+        "arrival_sampling_s": sample,
+        "cells_reached_fp64": int(np.isfinite(a64).sum()),
+        "cells_reached_fp32": int(np.isfinite(a32).sum()),
+        "volume_error_rel_fp32": res["fp32"]["vol_err"],
+        "volume_error_rel_fp64": res["fp64"]["vol_err"],
+        "iterations": {p: res[p]["iters"] for p in res},
+        "simulated_s": {p: res[p]["time"] for p in res},
+        "wall_s": {p: res[p]["wall"] for p in res},
+    }
+
+
 def cpu_baseline(spec, seconds, threads):
     """Time the FP64 oracle (OpenMP over rows) on the same case for about
     `seconds` of CPU work."""
@@ -316,6 +372,8 @@ def main():
                     help="length of the Poisson arrival window of the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hours", type=float, default=2.0)
+    ap.add_argument("--pond-break", type=int, default=4096,
+                    help="side of the pond-break precision study (FP32 form vs FP64; 0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -370,7 +428,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(args.precision, tile_rows, [R, C]),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by the 256 MiB probe "
+                              "(fetch x2.00, write x1.00); profiles/fused_step_2048_dropin.md",
             "bytes_per_cell_update": bpc,
             "kernel_avg_us": avg_s * 1e6,
             "working_set_MB": ws_mb,
@@ -378,10 +438,17 @@ def main():
         "volume_error_rel": bal["error"] / max(bal["volume"], 1e-30),
     }
 
+    h64 = case.field("h") if args.precision == "fp64" else None
     case.close()
     del case
     if args.fp32_steps > 0 and args.kernel == "fused" and args.precision == "fp64":
         c32, r32 = timed_run(pkg, spec, "fp32", "fused", 0, args.warmup, args.fp32_steps)
+        cmp32 = None
+        if args.fp32_steps == args.steps:
+            dh = c32.field("h") - h64
+            cmp32 = {"max_abs_dh_m": float(np.max(np.abs(dh))),
+                     "l2_dh_m": float(np.sqrt(np.mean(dh * dh))),
+                     "t_fp64_s": r["t_sim"], "t_fp32_s": r32["t_sim"]}
         c32.close()
         log(f"fp32: {r32['value']:.3e} cell-updates/s, tile rows {r32['tile_rows']}")
         out["fp32_form"] = {
@@ -392,6 +459,7 @@ def main():
                          "bytes_per_cell_update": r32["bpc"],
                          "kernel_avg_us": r32["avg_s"] * 1e6},
             "volume_error_rel": r32["balance"]["error"] / max(r32["balance"]["volume"], 1e-30),
+            "vs_fp64_same_iterations": cmp32,
         }
     if args.plain_steps > 0 and args.kernel == "fused" and args.precision == "fp64":
         plain = make_case(pkg, spec, "fp64", "plain", 0)
@@ -407,6 +475,10 @@ def main():
                              "steps": args.plain_steps,
                              "fused_speedup": value / pv}
         del plain
+
+    if args.pond_break > 0:
+        out["precision_study"] = pond_break_leg(pkg, cases, args.pond_break)
+        log(f"pond break: max|dh| {out['precision_study']['max_abs_dh_m']:.2e} m")
 
     plain_rates = {}
     if args.batch_scenarios > 0:

@@ -65,7 +65,7 @@ def build(verbose=False):
     dsrcs = [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
     if all(os.path.exists(s) for s in dsrcs):
         tmp = DRIVER + ".tmp"
-        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp, *dsrcs,
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-o", tmp, *dsrcs,
                f"-L{HERE}", "-lsw2d", f"-Wl,-rpath,$ORIGIN"]
         if verbose:
             print(" ".join(cmd))

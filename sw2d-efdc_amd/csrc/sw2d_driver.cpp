@@ -17,15 +17,72 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <condition_variable>
 #include <ctime>
+#include <deque>
+#include <functional>
+#include <mutex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sw2d.h"
 #include "sw2d_io.hpp"
 
 namespace {
+
+// Output writer thread.  Formatting the text outputs (VTK with "%f" per value,
+// WaterLevel.txt, QSER.INP) takes longer than the GPU needs for an output
+// interval on large grids, so the main thread hands each output to this
+// thread, with copies of the fields it writes, and goes on queueing the next
+// iterations.  Jobs run one at a time in submission order (the files come out
+// exactly as if written inline); at most `depth` jobs wait.
+class AsyncWriter {
+public:
+    explicit AsyncWriter(size_t depth) : depth_(depth), thr_([this] { run(); }) {}
+    ~AsyncWriter() { finish(); }
+    void submit(std::function<void()> job)
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        space_.wait(lk, [&] { return q_.size() < depth_; });
+        q_.push_back(std::move(job));
+        work_.notify_one();
+    }
+    // run every queued job, then stop the thread
+    void finish()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            work_.notify_all();
+        }
+        if (thr_.joinable()) thr_.join();
+    }
+
+private:
+    void run()
+    {
+        for (;;) {
+            std::function<void()> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                work_.wait(lk, [&] { return stop_ || !q_.empty(); });
+                if (q_.empty()) return;
+                job = std::move(q_.front());
+                q_.pop_front();
+                space_.notify_one();
+            }
+            job();
+        }
+    }
+    size_t depth_;
+    std::mutex mu_;
+    std::condition_variable work_, space_;
+    std::deque<std::function<void()>> q_;
+    bool stop_ = false;
+    std::thread thr_;  // last: starts after the members above exist
+};
 
 struct Options {
     std::string root = ".";
@@ -187,6 +244,7 @@ int run_case(const Options &opt, const std::string &name)
     bool efdc_note = false;
 
     std::vector<double> hbuf(RC), f1(RC), f2(RC);
+    AsyncWriter writer(2);
     double time = time0;
     long mstep = 0;
     int tq = -1;
@@ -226,32 +284,45 @@ int run_case(const Options &opt, const std::string &name)
             // coupling call (4913-4916)
             sw2d_get_field(h, SW2D_FIELD_UUA, uua_host.data());
             sw2d_get_field(h, SW2D_FIELD_VVA, vva_host.data());
-            sw2dio::write_qser(c.dir + "/EFDC/input/QSER.INP", 0, s.nlayers, qser, s.efdc_basetime, tq);
+            writer.submit([path = c.dir + "/EFDC/input/QSER.INP", nl = s.nlayers, snap = qser,
+                           base = s.efdc_basetime, tq] {
+                sw2dio::write_qser(path, 0, nl, snap, base, tq);
+            });
             if (tq == 0 && !efdc_note) {
                 printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
                 efdc_note = true;
             }
         }
-        sw2dio::VtkFields vf;
+        // this output's fields, copied for the writer thread
         const double *depth = (out0 == 0) ? c.h0.data() : hbuf.data();
-        if (s.out_depth == 1) {
-            vf.depth = depth;
-            if (s.out_outlet == 1)
-                sw2dio::write_water_level(wl, R, C, c.dem.xll / res, c.dem.yll / res, outx, outy, depth);
-        }
+        std::vector<double> d_copy, vv_copy, uu_copy;
+        if (s.out_depth == 1) d_copy.assign(depth, depth + RC);
         if (s.out_vx == 1) {
             sw2d_get_field(h, SW2D_FIELD_VV, f1.data());
-            vf.vv = f1.data();
+            vv_copy = f1;
         }
         if (s.out_vy == 1) {
             sw2d_get_field(h, SW2D_FIELD_UU, f2.data());
-            vf.uu = f2.data();
+            uu_copy = f2;
         }
-        if (s.out_elev == 1) vf.z = z.data();
-        sw2dio::write_vtk(outdir + "Results_" + tempo + ".vtk", C, R, res, res, c.dem.xll, c.dem.yll, vf);
+        const bool probes = s.out_depth == 1 && s.out_outlet == 1;
+        const double *zz = (s.out_elev == 1) ? z.data() : nullptr;
+        writer.submit([&, d = std::move(d_copy), vv = std::move(vv_copy), uu = std::move(uu_copy),
+                       path = outdir + "Results_" + tempo + ".vtk", probes, zz] {
+            if (probes)
+                sw2dio::write_water_level(wl, R, C, c.dem.xll / res, c.dem.yll / res, outx, outy,
+                                          d.data());
+            sw2dio::VtkFields vf;
+            vf.depth = d.empty() ? nullptr : d.data();
+            vf.vv = vv.empty() ? nullptr : vv.data();
+            vf.uu = uu.empty() ? nullptr : uu.data();
+            vf.z = zz;
+            sw2dio::write_vtk(path, C, R, res, res, c.dem.xll, c.dem.yll, vf);
+        });
     }
     if (pending > 0) check(sw2d_step(h, (int)pending), "step");
     sw2d_sync(h);
+    writer.finish();
     const double duration = (std::clock() - start) / (double)CLOCKS_PER_SEC;
     printf("printf: %g\n", duration);
     FILE *ts = fopen((outdir + "TimeSimu_" + tempo + ".txt").c_str(), "w");
