@@ -94,9 +94,9 @@ def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps):
     """Autotune (tile_rows 0), warm up, then time `steps` iterations with HIP
     events on the library's stream around every launch (or graph replay)."""
     case = make_case(pkg, spec, precision, kernel, tile_rows)
-    if kernel == "fused" and tile_rows == 0:
-        tile_rows = case.autotune(10)
     case.step(warmup)
+    if kernel == "fused" and tile_rows == 0:
+        tile_rows = case.autotune(10)  # on the warmed-up state
     case.sync()
     case.timing(True)
     case.sync()
@@ -351,7 +351,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3000)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=300)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--kernel", default="fused", choices=["fused", "plain"])

@@ -173,8 +173,10 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask);
 /* Extension (the project's autotuner): time the fused kernel for each
  * candidate tile height (rows per one-wave tile) over iters launches on the
  * current state and keep the fastest; *tile_rows receives it (0 for the
- * plain kernels).  Results do not depend on the tile height.  Only valid
- * before the first iteration.  Synchronises. */
+ * plain kernels).  The trial launches write only the spare state buffer and
+ * scratch bookkeeping, so it may run at any point of a run (the cost of a
+ * tile height depends on how much of the grid is wet); the state does not
+ * depend on the tile height.  Synchronises. */
 int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows);
 
 /* Advance n loop iterations of the reference's call sequence (4798-4877):

@@ -570,7 +570,6 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
 {
     if (check_idle(h)) return SW2D_E_ARG;
     if (iters < 1) return fail(h, SW2D_E_ARG, "iters must be positive");
-    if (h->step_index != 0) return fail(h, SW2D_E_STATE, "autotune runs before the first iteration");
     if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tuned by sw2d_batch_autotune");
     if (!fused(h)) {
         if (tile_rows) *tile_rows = 0;
@@ -610,13 +609,15 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
         HIPCHK(h, hipMemcpyAsync(clk_s, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, h->stream));
         HIPCHK(h, hipMemcpyAsync(desc_s, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
-        // every trial launch is iteration 0 again: it reads slot 0 of the
-        // scratch clock and buffer 0, and writes slot 1 and buffer 1
-        hipError_t e = launch_one(h, desc_s, 0);  // warm-up of this shape
+        // every trial launch is the next iteration k again: it reads clock
+        // slot k%3 of the scratch copy and the live buffer k%2, and writes
+        // only scratch bookkeeping and the spare buffer (k+1)%2
+        const int phase = (int)(h->step_index % 6);
+        hipError_t e = launch_one(h, desc_s, phase);  // warm-up of this shape
         if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
         HIPCHK(h, hipEventRecord(e0, h->stream));
         for (int i = 0; i < iters; i++)
-            if ((e = launch_one(h, desc_s, 0)) != hipSuccess) return hip_fail(h, e, "autotune launch");
+            if ((e = launch_one(h, desc_s, phase)) != hipSuccess) return hip_fail(h, e, "autotune launch");
         HIPCHK(h, hipEventRecord(e1, h->stream));
         HIPCHK(h, hipEventSynchronize(e1));
         float ms = 0;
@@ -627,12 +628,22 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     (void)hipEventDestroy(e1);
     dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
     set_tiles(h, best);
-    if (h->ntiles > h->nparts) {
+    // partial-sum records for the new tiling; the ones the last iteration
+    // wrote (reduced by the next) keep their positions, the rest are zero
+    const int old_n = h->nparts, new_n = std::max(old_n, h->ntiles);
+    if (new_n != old_n || best != H0) {
+        Partial *np = nullptr;
+        if ((rc = dalloc(h, &np, 2 * (size_t)new_n))) return rc;
+        if (h->step_index > 0) {
+            const size_t pend = (size_t)((h->step_index + 1) % 2);
+            HIPCHK(h, hipMemcpyAsync(np + pend * new_n, h->part + pend * old_n, sizeof(Partial) * old_n,
+                                     hipMemcpyDeviceToDevice, h->stream));
+        }
+        HIPCHK(h, hipStreamSynchronize(h->stream));
         dfree(h, h->part);
-        h->part = nullptr;
-        if ((rc = dalloc(h, &h->part, 2 * (size_t)h->ntiles))) return rc;
+        h->part = np;
+        h->nparts = new_n;
     }
-    h->nparts = h->ntiles;
     changed(h);
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (tile_rows) *tile_rows = best;

@@ -333,7 +333,7 @@ class Case:
         self.nbc = n.value
 
     def autotune(self, iters=20):
-        """Measured choice of the fused kernel's tile height (before stepping)."""
+        """Measured choice of the fused kernel's tile height on the current state."""
         best = ct.c_int(0)
         self._chk(self._L.sw2d_autotune(self._h, iters, ct.byref(best)), "autotune")
         return best.value

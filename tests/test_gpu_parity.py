@@ -209,3 +209,27 @@ def test_autotune_keeps_results(pkg, po, cases):
     compare(g, o, tag=f"autotuned H={H}")
     with pytest.raises(pkg.Sw2dError):
         g.autotune(3)            # only before the first iteration
+
+
+def test_autotune_mid_run_keeps_results(pkg, po, cases):
+    """Re-tuning the tile height after stepping has begun changes neither the
+    state (value for value against the oracle) nor the ledger bookkeeping."""
+    spec = cases.config0(rows=70, cols=131, seed=21, outlet=True)
+    g, o = pair(pkg, po, spec.z, spec.h0, spec.cellsize, tile_rows=0, adaptive=True)
+    for c in (g, o):
+        c.set_rain(spec.rain, 600.0)
+        c.set_outlet(*spec.outlet)
+    o.set_adaptive(0.5, 1.0)
+    g.set_hydrograph(400)
+    o.set_hydrograph(400)
+    g.step(7)
+    g.autotune(2)   # odd iteration: the live state is in the second buffer
+    g.step(30)
+    g.autotune(2)
+    g.step(13)
+    o.step(50)
+    compare(g, o, tag="mid-run autotune")
+    bg, bo = g.balance(), o.ledger()
+    assert bg["volume"] == pytest.approx(bo["volume"], rel=1e-12)
+    assert bg["v_outflow"] == pytest.approx(bo["v_out"], rel=1e-12, abs=1e-9)
+    assert np.array_equal(g.hydrograph()[:, 1], o.hydrograph()[:, 1])
