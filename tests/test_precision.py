@@ -99,3 +99,34 @@ def test_fp32_volume_ledger_rain_ensemble(pkg, cases):
     print("FP32 volume errors:", ["%.1e" % e for e in errs])
     assert max(errs) <= 1e-6
     batch.close()
+
+
+def test_fp32_form_on_ensemble_scenarios(pkg, cases):
+    """max|dh| of the FP32 form against FP64 (which equals the oracle value for
+    value) on ensemble scenarios of the batched/served workload: 1 h block
+    storms on rough slopes, dx 10 m, fixed dt."""
+    b32 = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
+    b64 = pkg.Batch(precision="fp64", pool_bytes=512 << 20)
+    pairs = []
+    for k, side in ((3, 128), (11, 200), (19, 256)):
+        spec = cases.ensemble_scenario(k, side=side)
+        ms = []
+        for b in (b32, b64):
+            m = b.case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+            m.set_rain(spec.rain, spec.dtrain)
+            m.set_outlet(*spec.outlet)
+            m.set_hydrograph(4000)
+            b.attach(m, t_end=spec.t_end)
+            ms.append(m)
+        pairs.append(ms)
+    while b32.step(120) + b64.step(120):
+        pass
+    for m32, m64 in pairs:
+        dh = np.abs(m32.field("h") - m64.field("h"))
+        q32, q64 = m32.balance()["q_peak"], m64.balance()["q_peak"]
+        print(f"ensemble {m32.R}x{m32.C}: max|dh| {dh.max():.2e} m, L2 {np.sqrt(np.mean(dh**2)):.2e} m, "
+              f"peak outflow {q32:.4f} vs {q64:.4f} m3/s")
+        assert dh.max() <= 1e-3
+        assert abs(q32 - q64) <= 1e-3 * q64
+    b32.close()
+    b64.close()
