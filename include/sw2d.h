@@ -179,6 +179,11 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask);
  * depend on the tile height.  Synchronises. */
 int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows);
 
+/* Extension: set the fused kernel's rows per tile (1..512) at any point of a
+ * run, as sw2d_autotune does with its choice.  No effect on the plain
+ * kernels; the state does not depend on it. */
+int sw2d_set_tile_rows(sw2d_handle *h, int tile_rows);
+
 /* Advance n loop iterations of the reference's call sequence (4798-4877):
  * flux, continuity(t), treat_error, continuity(t + dt), derived fields.
  * Asynchronous: returns once the work is queued. */

@@ -131,6 +131,39 @@ void advance(sw2d_handle *h, long m)
     h->cur = (int)(h->step_index & 1);
 }
 
+// Switch a fused handle to H rows per tile.  The partial sums the last
+// iteration wrote are still to be reduced by the next one; they are folded
+// into one record of the new buffer (sized for the new tiling exactly), so no
+// record of the old tiling is ever reduced twice.
+int retile(sw2d_handle *h, int H)
+{
+    const int old_n = h->nparts;
+    const int H0 = h->H;
+    set_tiles(h, H);
+    const int new_n = h->ntiles;
+    if (new_n == old_n && H == H0) return SW2D_OK;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    Partial *np = nullptr;
+    int rc = dalloc(h, &np, 2 * (size_t)new_n);
+    if (rc) return rc;
+    if (h->step_index > 0) {
+        const size_t pend = (size_t)((h->step_index + 1) % 2);
+        std::vector<Partial> old((size_t)old_n);
+        HIPCHK(h, hipMemcpy(old.data(), h->part + pend * old_n, sizeof(Partial) * old_n,
+                            hipMemcpyDeviceToHost));
+        Partial sum{0.0, 0.0, 0.0, 0.0};
+        for (const Partial &p : old) {
+            sum.vol += p.vol; sum.src += p.src; sum.clip += p.clip; sum.qout += p.qout;
+        }
+        HIPCHK(h, hipMemcpy(np + pend * new_n, &sum, sizeof sum, hipMemcpyHostToDevice));
+    }
+    dfree(h, h->part);
+    h->part = np;
+    h->nparts = new_n;
+    changed(h);
+    return SW2D_OK;
+}
+
 int flush_events(sw2d_handle *h)
 {
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -627,27 +660,20 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
-    set_tiles(h, best);
-    // partial-sum records for the new tiling; the ones the last iteration
-    // wrote (reduced by the next) keep their positions, the rest are zero
-    const int old_n = h->nparts, new_n = std::max(old_n, h->ntiles);
-    if (new_n != old_n || best != H0) {
-        Partial *np = nullptr;
-        if ((rc = dalloc(h, &np, 2 * (size_t)new_n))) return rc;
-        if (h->step_index > 0) {
-            const size_t pend = (size_t)((h->step_index + 1) % 2);
-            HIPCHK(h, hipMemcpyAsync(np + pend * new_n, h->part + pend * old_n, sizeof(Partial) * old_n,
-                                     hipMemcpyDeviceToDevice, h->stream));
-        }
-        HIPCHK(h, hipStreamSynchronize(h->stream));
-        dfree(h, h->part);
-        h->part = np;
-        h->nparts = new_n;
-    }
-    changed(h);
+    set_tiles(h, H0);
+    if ((rc = retile(h, best))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (tile_rows) *tile_rows = best;
     return SW2D_OK;
+}
+
+int sw2d_set_tile_rows(sw2d_handle *h, int tile_rows)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (tile_rows < 1 || tile_rows > 512) return fail(h, SW2D_E_ARG, "tile_rows must be in [1, 512]");
+    if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tiled by their batch");
+    if (!fused(h)) return SW2D_OK;
+    return retile(h, tile_rows);
 }
 
 int sw2d_step(sw2d_handle *h, int n_iters)

@@ -87,6 +87,7 @@ int fused_blocks(const sw2d_handle *h);
 Scenario make_scenario(sw2d_handle *h);
 int choose_tile_rows(const sw2d_handle *h);
 void set_tiles(sw2d_handle *h, int H);
+int retile(sw2d_handle *h, int H);
 void advance(sw2d_handle *h, long m);
 int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
                 const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out);

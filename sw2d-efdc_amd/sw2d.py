@@ -112,6 +112,7 @@ def lib():
         "sw2d_set_hydrograph": (ct.c_int, [_H, ct.c_long]),
         "sw2d_set_lake_bc": (ct.c_int, [_H, _IP]),
         "sw2d_autotune": (ct.c_int, [_H, ct.c_int, _IP]),
+        "sw2d_set_tile_rows": (ct.c_int, [_H, ct.c_int]),
         "sw2d_step": (ct.c_int, [_H, ct.c_int]),
         "sw2d_run_until": (ct.c_int, [_H, _D, ct.c_long, ct.POINTER(ct.c_long)]),
         "sw2d_sync": (ct.c_int, [_H]),
@@ -337,6 +338,9 @@ class Case:
         best = ct.c_int(0)
         self._chk(self._L.sw2d_autotune(self._h, iters, ct.byref(best)), "autotune")
         return best.value
+
+    def set_tile_rows(self, rows):
+        self._chk(self._L.sw2d_set_tile_rows(self._h, rows), "set_tile_rows")
 
     def step(self, n=1):
         self._chk(self._L.sw2d_step(self._h, n), "step")
