@@ -202,14 +202,15 @@ def batch_leg(pkg, cases, n, plain_rates):
     }
 
 
-def serve_leg(pkg, cases, seconds, rate, hours, plain_rates):
+def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0):
     """A bounded serving session: Poisson arrivals at `rate`/s for `seconds`,
     grid sides 256..4096 (p 0.4/0.3/0.15/0.1/0.05), 2 h storms, FP32 form."""
     n = max(1, int(round(seconds * rate)))
     arrivals, specs = cases.serve_requests(n, rate=rate, t_end=3600.0 * hours)
     log(f"serving: {n} requests generated; submitting")
     tile = 32
-    srv = pkg.Server(precision="fp32", pool_bytes=64 << 30, tile_rows=tile, slice_iters=60)
+    srv = pkg.Server(precision="fp32", pool_bytes=64 << 30, tile_rows=tile, slice_iters=60,
+                     max_active_cells=active_cells)
     cap = int(3600.0 * hours / (2 * 0.5)) + 16
     ids = []
     t0 = time.perf_counter()
@@ -240,7 +241,8 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates):
     return {
         "workload": f"{n} requests, Poisson {rate}/s over {seconds} s, sides "
                     f"{sorted(set(sides))} (p 0.4/0.3/0.15/0.1/0.05), {hours} h storms, dx 10 m, "
-                    "fixed dt 0.5 s, FP32 state / FP64 balance, HBM-footprint admission",
+                    "fixed dt 0.5 s, FP32 state / FP64 balance, admission by HBM footprint and "
+                    f"at most {active_cells} running cells",
         "requests": n,
         "completed": st["completed"],
         "failed": st["failed"],
@@ -372,6 +374,8 @@ def main():
                     help="length of the Poisson arrival window of the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hours", type=float, default=2.0)
+    ap.add_argument("--serve-active-cells", type=int, default=32 << 20,
+                    help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--pond-break", type=int, default=4096,
                     help="side of the pond-break precision study (FP32 form vs FP64; 0 = skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -486,7 +490,7 @@ def main():
         log(f"batch: {out['batch']['value']:.3e} cell-updates/s")
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(pkg, cases, args.serve_seconds, args.serve_rate,
-                                   args.serve_hours, plain_rates)
+                                   args.serve_hours, plain_rates, args.serve_active_cells)
         log(f"serving: p50 {out['serving']['p50_time_to_hydrograph_s']:.2f} s, "
             f"p99 {out['serving']['p99_time_to_hydrograph_s']:.2f} s")
 

@@ -357,9 +357,13 @@ typedef struct sw2d_server_stats {
     double kernel_ms;                /* HIP-event time of the step launches */
 } sw2d_server_stats;
 
-/* slice_iters: iterations per batch step between admissions (0 = 60). */
+/* slice_iters: iterations per batch step between admissions (0 = 60).
+ * max_active_cells: admission also stops while the running scenarios hold
+ * this many cells (0 = no limit; a request is always admitted onto an idle
+ * GPU).  With enough cells in flight to fill the GPU, admitting more only
+ * shares it among more requests: the same throughput, later results. */
 int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_rows, int slice_iters,
-                       sw2d_server **out);
+                       long max_active_cells, sw2d_server **out);
 /* Copies the request's arrays; returns its id. */
 int sw2d_server_submit(sw2d_server *s, const sw2d_request *req, long *id);
 int sw2d_server_poll(sw2d_server *s, long id, sw2d_result *out);

@@ -54,6 +54,8 @@ double percentile(std::vector<double> v, double p)
 
 struct sw2d_server {
     int device = 0, precision = SW2D_FP32, tile_rows = 32, slice = 60;
+    long max_active = 0;       // cell budget of the running set (0 = none)
+    long active_cells = 0;     // cells of the running scenarios
     sw2d_batch *batch = nullptr;
     std::thread thr;
     std::mutex mu;
@@ -72,6 +74,8 @@ struct sw2d_server {
     std::vector<double> lat, wait;
     std::chrono::steady_clock::time_point t0;
     std::string err;
+
+    static long cells_of(const Request *r) { return (long)r->req.grid.rows * r->req.grid.cols; }
 
     double now() const
     {
@@ -127,6 +131,7 @@ struct sw2d_server {
         sw2d_destroy(h);
         std::lock_guard<std::mutex> lk(mu);
         r->h = nullptr;
+        active_cells -= cells_of(r);
         r->res.iterations = iters;
         r->res.time = t;
         r->res.t_done = now();
@@ -162,7 +167,10 @@ struct sw2d_server {
                     r = queue.front();
                     size_t used = 0, cap = 0;
                     sw2d_batch_memory(batch, &used, nullptr, &cap);
-                    if (r->footprint > cap - used && !running.empty()) break;
+                    if (!running.empty()) {
+                        if (r->footprint > cap - used) break;  // HBM footprint
+                        if (max_active > 0 && active_cells + cells_of(r) > max_active) break;
+                    }
                     queue.pop_front();
                 }
                 const int rc = admit(r);
@@ -180,6 +188,7 @@ struct sw2d_server {
                 }
                 r->res.state = SW2D_REQ_RUNNING;
                 r->res.t_admit = now();
+                active_cells += cells_of(r);
                 running.push_back(r);
             }
             if (running.empty()) {
@@ -234,7 +243,7 @@ struct sw2d_server {
 extern "C" {
 
 int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_rows, int slice_iters,
-                       sw2d_server **out)
+                       long max_active_cells, sw2d_server **out)
 {
     if (!out) return SW2D_E_ARG;
     *out = nullptr;
@@ -247,6 +256,7 @@ int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_ro
     s->precision = precision;
     s->tile_rows = tile_rows > 0 ? tile_rows : 32;
     s->slice = slice_iters > 0 ? slice_iters : 60;
+    s->max_active = max_active_cells > 0 ? max_active_cells : 0;
     int rc = sw2d_batch_create(device, precision, pool_bytes, s->tile_rows, &s->batch);
     if (rc) {
         delete s;
@@ -363,7 +373,9 @@ int sw2d_server_wait(sw2d_server *s, long id, double timeout_s)
             const int st = it->second->res.state;
             return st == SW2D_REQ_DONE || st == SW2D_REQ_FAILED;
         }
-        return s->queue.empty() && s->running.empty();
+        // every submitted request has finished; the queue and the running set
+        // are both empty for a moment while the device thread admits a request
+        return s->completed + s->failed == s->submitted;
     };
     const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
     if (!ok) {

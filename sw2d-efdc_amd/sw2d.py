@@ -145,7 +145,7 @@ def lib():
         "sw2d_batch_timing": (ct.c_int, [_H, ct.c_int]),
         "sw2d_batch_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
         "sw2d_server_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.c_int,
-                                          ct.POINTER(_H)]),
+                                          ct.c_long, ct.POINTER(_H)]),
         "sw2d_server_submit": (ct.c_int, [_H, ct.POINTER(Request), ct.POINTER(ct.c_long)]),
         "sw2d_server_poll": (ct.c_int, [_H, ct.c_long, ct.POINTER(Result)]),
         "sw2d_server_hydrograph": (ct.c_int, [_H, ct.c_long, _DP, ct.c_long, ct.POINTER(ct.c_long)]),
@@ -480,11 +480,12 @@ class Server:
     """Continuous-batching request loop on a device thread (sw2d_server_*)."""
 
     def __init__(self, precision="fp32", pool_bytes=8 << 30, tile_rows=32, slice_iters=60,
-                 device=0):
+                 max_active_cells=0, device=0):
         self._L = lib()
         self._s = _H()
         rc = self._L.sw2d_server_create(device, {"fp64": FP64, "fp32": FP32}[precision],
-                                        pool_bytes, tile_rows, slice_iters, ct.byref(self._s))
+                                        pool_bytes, tile_rows, slice_iters, max_active_cells,
+                                        ct.byref(self._s))
         if rc:
             raise Sw2dError(f"sw2d_server_create failed ({rc}): "
                             f"{self._L.sw2d_server_last_error(None).decode()}")

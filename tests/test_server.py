@@ -93,3 +93,24 @@ def test_request_too_large_fails_cleanly(pkg):
     srv.wait(rid2, timeout=60)
     assert srv.poll(rid2)["state"] == "done"
     srv.close()
+
+
+def test_active_cell_cap_orders_completion(pkg, cases):
+    """With a cap on running cells the server admits the queue in arrival order
+    as earlier requests finish; results stay those of the stand-alone runs."""
+    reqs = [request(cases, i) for i in (0, 4, 5)]
+    cells = [z.size for z, *_ in reqs]
+    srv = pkg.Server(precision="fp32", pool_bytes=256 << 20, tile_rows=H, slice_iters=12,
+                     max_active_cells=max(cells) + 1)
+    ids = [srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                      hydro_cap=CAP, **kw) for z, outlet, rain, kw, t_end in reqs]
+    srv.wait(timeout=300)
+    res = [srv.poll(i) for i in ids]
+    assert all(r["state"] == "done" for r in res)
+    # one at a time: each was admitted no earlier than the previous one finished
+    for a, b in zip(res, res[1:]):
+        assert b["t_admit"] >= a["t_done"]
+    for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
+        hq, bal = alone(pkg, z, outlet, rain, kw, t_end)
+        assert np.array_equal(srv.hydrograph(ids[i]), hq)
+    srv.close()
