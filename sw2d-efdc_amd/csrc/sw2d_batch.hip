@@ -343,8 +343,19 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     for (sw2d_handle *h : b->members)
         if (h->step_index != 0) return bfail(b, SW2D_E_STATE, "autotune runs before the first iteration");
     BCHK(b, hipStreamSynchronize(b->stream));
-    const int cand[] = {8, 12, 16, 24, 32, 48, 64, 96, 128};
     const int nm = (int)b->members.size();
+    std::vector<int> cand = {8, 12, 16, 24, 32, 48, 64, 96, 128};
+    // heights whose launch over all members fills whole rounds of wave slots
+    const long slots = wave_slots(b->device, b->members[0]->esize);
+    for (int H : round_fill_heights(slots, [&](int H) {
+             long n = 0;
+             for (const sw2d_handle *h : b->members)
+                 n += blocks_at(h, h->prm.tile_rows > 0 ? h->prm.tile_rows : H);
+             return n;
+         }))
+        cand.push_back(H);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     // scratch bookkeeping per member, sized for the smallest candidate
     struct Scratch { Clock *clk; Partial *part; Ledger *led; double *bc; int maxt; };
     std::vector<Scratch> sc(nm);

@@ -5,7 +5,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
+#include <vector>
 
 #include "sw2d_internal.hpp"
 
@@ -86,6 +88,9 @@ bool fused(const sw2d_handle *h);
 int fused_blocks(const sw2d_handle *h);
 Scenario make_scenario(sw2d_handle *h);
 int choose_tile_rows(const sw2d_handle *h);
+long wave_slots(int device, int esize);
+int blocks_at(const sw2d_handle *h, int H);
+std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &blocks);
 void set_tiles(sw2d_handle *h, int H);
 int retile(sw2d_handle *h, int H);
 void advance(sw2d_handle *h, long m);
