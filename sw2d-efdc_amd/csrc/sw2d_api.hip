@@ -64,6 +64,7 @@ Scenario make_scenario(sw2d_handle *h)
     Scenario d;
     memset(&d, 0, sizeof d);
     d.s = make_common(h);
+    h->features = features_of(d.s);
     for (int b = 0; b < 2; b++) {
         d.st[b][0] = h->h[b];
         d.st[b][1] = h->um[b];
@@ -96,8 +97,8 @@ int upload_desc(sw2d_handle *h)
 hipError_t launch_one(sw2d_handle *h, const Scenario *scen, int phase)
 {
     return h->esize == 8
-               ? launch_fused_f64(h->stream, scen, nullptr, fused_blocks(h), phase)
-               : launch_fused_f32(h->stream, scen, nullptr, fused_blocks(h), phase);
+               ? launch_fused_f64(h->stream, scen, nullptr, fused_blocks(h), phase, h->features)
+               : launch_fused_f32(h->stream, scen, nullptr, fused_blocks(h), phase, h->features);
 }
 
 // the graph of iterations k .. k+5 (k % 6 == 0): six launches, phases 0..5
@@ -180,9 +181,9 @@ int flush_events(sw2d_handle *h)
 
 // Resident one-wave blocks of the fused kernel over the device (waves per CU
 // at its register use x CUs): one "round" of a launch.
-long wave_slots(int device, int esize)
+long wave_slots(int device, int esize, int features)
 {
-    const int per_cu = esize == 8 ? fused_waves_per_cu_f64() : fused_waves_per_cu_f32();
+    const int per_cu = esize == 8 ? fused_waves_per_cu_f64(features) : fused_waves_per_cu_f32(features);
     hipDeviceProp_t prop;
     int cus = 0;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
@@ -221,7 +222,7 @@ std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &
 int choose_tile_rows(const sw2d_handle *h)
 {
     if (h->prm.tile_rows > 0) return std::min(h->prm.tile_rows, 512);
-    const long slots = wave_slots(h->device, h->esize);
+    const long slots = wave_slots(h->device, h->esize, features_of(make_common(const_cast<sw2d_handle *>(h))));
     const std::vector<int> fit = round_fill_heights(slots, [&](int H) { return (long)blocks_at(h, H); });
     for (auto it = fit.rbegin(); it != fit.rend(); ++it)
         if (*it <= 48) return *it;
@@ -642,7 +643,7 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     }
     HIPCHK(h, hipStreamSynchronize(h->stream));
     std::vector<int> cand = {8, 10, 12, 14, 16, 18, 20, 24, 28, 32, 40, 48, 64, 96, 128};
-    for (int H : round_fill_heights(wave_slots(h->device, h->esize),
+    for (int H : round_fill_heights(wave_slots(h->device, h->esize, h->features),
                                     [&](int H) { return (long)blocks_at(h, H); }))
         cand.push_back(H);
     std::sort(cand.begin(), cand.end());

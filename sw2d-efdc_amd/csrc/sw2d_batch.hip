@@ -43,10 +43,11 @@ void bdrop_graph(sw2d_batch *b)
     b->gexec = nullptr;
 }
 
-hipError_t blaunch(sw2d_batch *b, const Scenario *scen, const TileRef *tiles, int nblocks, int phase)
+hipError_t blaunch(sw2d_batch *b, const Scenario *scen, const TileRef *tiles, int nblocks, int phase,
+                   int features)
 {
-    return b->esize == 8 ? launch_fused_f64(b->stream, scen, tiles, nblocks, phase)
-                         : launch_fused_f32(b->stream, scen, tiles, nblocks, phase);
+    return b->esize == 8 ? launch_fused_f64(b->stream, scen, tiles, nblocks, phase, features)
+                         : launch_fused_f32(b->stream, scen, tiles, nblocks, phase, features);
 }
 
 // descriptors and tile table of the attached members
@@ -60,8 +61,10 @@ int rebuild(sw2d_batch *b)
     for (sw2d_handle *h : b->members) top = std::max(top, h->member_slot + 1);
     desc.resize(top);
     memset(desc.data(), 0, sizeof(Scenario) * desc.size());
+    int features = 0;
     for (sw2d_handle *h : b->members) {
         Scenario d = make_scenario(h);
+        features |= features_of(d.s);
         d.phase0 = h->phase0;
         d.k_end = h->k_end;
         d.t_end = h->t_end;
@@ -83,8 +86,9 @@ int rebuild(sw2d_batch *b)
         BCHK(b, hipMemcpy(b->dscen, desc.data(), sizeof(Scenario) * top, hipMemcpyHostToDevice));
     if (!tiles.empty())
         BCHK(b, hipMemcpy(b->dtiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice));
-    if ((int)tiles.size() != b->nblocks) bdrop_graph(b);
+    if ((int)tiles.size() != b->nblocks || features != b->features) bdrop_graph(b);
     b->nblocks = (int)tiles.size();
+    b->features = features;
     b->dirty = false;
     return SW2D_OK;
 }
@@ -95,7 +99,7 @@ int ensure_bgraph(sw2d_batch *b)
     hipGraph_t g = nullptr;
     BCHK(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int ph = 0; ph < 6 && e == hipSuccess; ph++) e = blaunch(b, b->dscen, b->dtiles, b->nblocks, ph);
+    for (int ph = 0; ph < 6 && e == hipSuccess; ph++) e = blaunch(b, b->dscen, b->dtiles, b->nblocks, ph, b->features);
     hipError_t e2 = hipStreamEndCapture(b->stream, &g);
     if (e != hipSuccess) { if (g) (void)hipGraphDestroy(g); return bhip(b, e, "graph capture launch"); }
     if (e2 != hipSuccess) return bhip(b, e2, "hipStreamEndCapture");
@@ -308,7 +312,7 @@ int sw2d_batch_step(sw2d_batch *b, int n_iters, int *running)
             if (group) {
                 BCHK(b, hipGraphLaunch(b->gexec, b->stream));
             } else {
-                hipError_t e = blaunch(b, b->dscen, b->dtiles, b->nblocks, (int)(b->phase % 6));
+                hipError_t e = blaunch(b, b->dscen, b->dtiles, b->nblocks, (int)(b->phase % 6), b->features);
                 if (e != hipSuccess) return bhip(b, e, "batch launch");
             }
             if (b->timing) {
@@ -346,7 +350,9 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     const int nm = (int)b->members.size();
     std::vector<int> cand = {8, 12, 16, 24, 32, 48, 64, 96, 128};
     // heights whose launch over all members fills whole rounds of wave slots
-    const long slots = wave_slots(b->device, b->members[0]->esize);
+    int feats_all = 0;
+    for (sw2d_handle *h : b->members) feats_all |= features_of(make_common(h));
+    const long slots = wave_slots(b->device, b->esize, feats_all);
     for (int H : round_fill_heights(slots, [&](int H) {
              long n = 0;
              for (const sw2d_handle *h : b->members)
@@ -396,6 +402,7 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     for (int H : cand) {
         std::vector<Scenario> desc(nm);
         std::vector<TileRef> tiles;
+        int feats = 0;
         for (int i = 0; i < nm && e == hipSuccess; i++) {
             sw2d_handle *h = b->members[i];
             const int H0 = h->H;
@@ -409,6 +416,7 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
             d.s.hydro = nullptr;
             d.s.bc_acc = sc[i].bc;
             d.phase0 = 0;
+            feats |= features_of(d.s);
             desc[i] = d;
             for (int k = 0; k < d.nblocks; k++) tiles.push_back(TileRef{i, k});
             e = hipMemcpyAsync(sc[i].clk, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, b->stream);
@@ -417,9 +425,9 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
         if (e == hipSuccess) e = hipMemcpy(dtiles_s, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) break;
         // every trial launch is iteration 0 of every member again
-        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0);
+        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0, feats);
         if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0);
+        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0, feats);
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;

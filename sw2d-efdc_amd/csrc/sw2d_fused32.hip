@@ -1,23 +1,19 @@
 // sw2d_fused32.hip -- FP32 instantiation of the fused SW2D kernel (precision
 // form: FP32 state, fluxes and local arithmetic; FP64 bed elevation, water
 // surfaces, time, CFL minimum, volume ledger and hydrograph).  Built with
-// -ffp-contract=fast: FP32 multiply-adds are fused.
+// -ffp-contract=on: a multiply-add is fused only within one source
+// expression, so the arithmetic does not depend on the code around it (the
+// kernel's instantiations give a scenario the same results).
 #include "sw2d_fused_kernel.hpp"
 
 namespace sw2d {
 
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase)
+                           int phase, int features)
 {
-    return launch_fused_impl<float>(stream, scen, tiles, nblocks, phase);
+    return launch_fused_impl<float>(stream, scen, tiles, nblocks, phase, features);
 }
-int fused_waves_per_cu_f32()
-{
-    int nb = 0;
-    const void *fn = (const void *)fused_step<float>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
-    return nb;
-}
+int fused_waves_per_cu_f32(int features) { return fused_waves_per_cu_impl<float>(features); }
 hipError_t launch_init_f32(sw2d_handle *h) { return launch_init_impl<float>(h); }
 hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out) { return launch_derive_impl<float>(h, field, out); }
 

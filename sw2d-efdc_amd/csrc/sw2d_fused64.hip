@@ -11,17 +11,11 @@ __global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *ou
 }
 
 hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase)
+                           int phase, int features)
 {
-    return launch_fused_impl<double>(stream, scen, tiles, nblocks, phase);
+    return launch_fused_impl<double>(stream, scen, tiles, nblocks, phase, features);
 }
-int fused_waves_per_cu_f64()
-{
-    int nb = 0;
-    const void *fn = (const void *)fused_step<double>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
-    return nb;
-}
+int fused_waves_per_cu_f64(int features) { return fused_waves_per_cu_impl<double>(features); }
 hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double>(h); }
 hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out) { return launch_derive_impl<double>(h, field, out); }
 

@@ -43,12 +43,20 @@ __device__ __forceinline__ bool west_bit(bool p, int lane)
 
 constexpr int kMaxTileRows = 512;
 
-template <typename T>
+// F: the features (kFeat*) this instantiation handles.  A feature missing
+// from F is compiled out (its uniforms, branches and loops), which the host
+// may only do for launches where no scenario uses it.  The arithmetic of
+// what remains is the same in every instantiation (FP64 fuses no
+// multiply-add, FP32 only within one source expression), so a scenario's
+// results do not depend on the instantiation that ran it.
+template <typename T, int F>
 __global__ void __launch_bounds__(64)
 fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles, const int nblocks,
            const int phase)
 {
     using M = num<T>;
+    constexpr bool kStorm = F & kFeatStorm, kSources = F & kFeatSources;
+    constexpr bool kAdaptive = F & kFeatAdaptive, kMap = F & kFeatMap;
     const int lane = threadIdx.x;
 
     // block -> (scenario, block within it).  The hardware deals consecutive
@@ -133,7 +141,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
 
     // CFL candidates only for an adaptive time step; a wave-uniform flag, so
     // a scenario runs the same instructions alone and in a mixed batch
-    const bool adapt = !s.fixed_dt;
+    const bool adapt = kAdaptive && !s.fixed_dt;
+    const bool storm_on = kStorm && s.storm_on;
+    const bool map_on = kMap && s.ncls_on;
     // FP32 form: water-surface tests from bed differences carried per face
     // (z stays FP64 only to form them), storm factors and source terms in FP32
     constexpr bool kF32 = sizeof(T) == 4;
@@ -150,7 +160,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     double gx1 = 0, gx2 = 0;
     T pk1 = 0, pk2 = 0;  // FP32 form: peak * gx of this column, 0 outside the storm window
     __shared__ T gy_s[2][kMaxTileRows + 8];
-    if (s.storm_on) {
+    if (storm_on) {
         const double x = ((double)j + 0.5) * s.dx;
         gx1 = storm_factor(x - (s.st_x0 + s.st_vx * st.t), s.st_sigma);
         gx2 = storm_factor(x - (s.st_x0 + s.st_vx * st.tm), s.st_sigma);
@@ -204,7 +214,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         uu = in ? lu : T(0);
         vv = in ? lv : T(0);
         zz = in ? lz : __builtin_nan("");
-        cc = (s.ncls_on && in) ? clsg[off] : (uint8_t)0;
+        cc = (map_on && in) ? clsg[off] : (uint8_t)0;
     };
     T nh, nu, nv;
     double nz;
@@ -217,7 +227,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
-        const T nc0 = s.ncls_on ? (T)s.ntab[cl0] : rn;
+        const T nc0 = map_on ? (T)s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
         const T h0w = from_west(h0);
@@ -309,7 +319,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
                 const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
                 const T u12 = (j != 0) ? dt2dy * (umu - umd) : T(0);
-                const T nf = s.ncls_on ? T(0.5) * (nc2 + nc1) : rn;
+                const T nf = map_on ? T(0.5) * (nc2 + nc1) : rn;
                 T ram = 0.00;
                 if (b4 && !(hm1 <= th)) {
                     const T sqx = M::sqrt_(uu1_1 * uu1_1 + vva1 * vva1);
@@ -366,7 +376,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 const T vnu = pv * (v1e + v1) * T(0.50) + M::abs_(pv) * (v1 - v1e) * T(0.50);
                 const T vnd = qv * (v1w + v1) * T(0.50) + M::abs_(qv) * (v1w - v1) * T(0.50);
                 const T v12 = dt2dy * (vnu - vnd);
-                const T nf = s.ncls_on ? T(0.5) * (nc1w + nc1) : rn;
+                const T nf = map_on ? T(0.5) * (nc1w + nc1) : rn;
                 T ram = 0.00;
                 if (b4 && !(hn1 <= th)) {
                     const T sqy = M::sqrt_(uua1 * uua1 + vv1_1 * vv1_1);
@@ -389,14 +399,14 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             const T Pc = wet ? P_lake : P_land;
             const T E = wet ? ev1t : T(0.000);
             T rr = rr1t;
-            if (s.storm_on) {
+            if (storm_on) {
                 const int gi = g - (r0 - 5);
                 const T gy = gy_s[0][gi < 0 ? 0 : gi];
                 if constexpr (kF32) rr = pk1 * gy;
                 else rr = storm1 ? (T)(s.st_peak * (gx1 * gy)) : T(0);
             }
             T ql = 0;
-            for (int i = 0; i < s.nst; ++i)
+            for (int i = 0; kSources && i < s.nst; ++i)
                 if (s.src_r[i] == g && s.src_c[i] == j) ql = (T)source_rate(s, i, st.t);
             T ums = umn, umnn = umnP, vne = vnnPe, vnw = vnnP;
             if (is_out_col && g == s.out_r) {
@@ -431,14 +441,14 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             const T Pc = wet ? P_lake : P_land;
             const T E = wet ? ev2t : T(0.000);
             T rr = rr2t;
-            if (s.storm_on) {
+            if (storm_on) {
                 const int qi = q - (r0 - 6);
                 const T gy = gy_s[1][qi < 0 ? 0 : qi];
                 if constexpr (kF32) rr = pk2 * gy;
                 else rr = storm2 ? (T)(s.st_peak * (gx2 * gy)) : T(0);
             }
             T ql = 0;
-            for (int i = 0; i < s.nst; ++i)
+            for (int i = 0; kSources && i < s.nst; ++i)
                 if (s.src_r[i] == q && s.src_c[i] == j) ql = (T)source_rate(s, i, st.tm);
             T ums = umpg, umnn = umpP, vne = vnpPe, vnw = vnpP;
             const bool outcell = is_out_col && q == s.out_r;
@@ -605,10 +615,43 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
 
 template <typename T>
 static hipError_t launch_fused_impl(hipStream_t stream, const Scenario *scen, const TileRef *tiles,
-                                    int nblocks, int phase)
+                                    int nblocks, int phase, int features)
 {
-    hipLaunchKernelGGL(fused_step<T>, dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+    switch (fused_variant(features)) {
+    case kVariant0:
+        hipLaunchKernelGGL((fused_step<T, kVariant0>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        break;
+    case kVariant1:
+        hipLaunchKernelGGL((fused_step<T, kVariant1>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        break;
+    case kVariant2:
+        hipLaunchKernelGGL((fused_step<T, kVariant2>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        break;
+    case kVariant3:
+        hipLaunchKernelGGL((fused_step<T, kVariant3>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        break;
+    default:
+        hipLaunchKernelGGL((fused_step<T, kFeatAll>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        break;
+    }
     return hipGetLastError();
+}
+
+// resident one-wave blocks per CU of the instantiation that runs `features`
+template <typename T>
+static int fused_waves_per_cu_impl(int features)
+{
+    const void *fn;
+    switch (fused_variant(features)) {
+    case kVariant0: fn = (const void *)fused_step<T, kVariant0>; break;
+    case kVariant1: fn = (const void *)fused_step<T, kVariant1>; break;
+    case kVariant2: fn = (const void *)fused_step<T, kVariant2>; break;
+    case kVariant3: fn = (const void *)fused_step<T, kVariant3>; break;
+    default: fn = (const void *)fused_step<T, kFeatAll>; break;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
+    return nb;
 }
 
 template <typename T>

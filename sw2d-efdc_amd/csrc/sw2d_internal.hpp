@@ -134,6 +134,33 @@ struct TileRef {
     int sid, local;
 };
 
+// Features of a scenario that the fused kernel can compile out where no
+// scenario of a launch uses them.
+enum : int {
+    kFeatStorm = 1,     // moving storm (sw2d_set_storm)
+    kFeatSources = 2,   // point inflow series (stream_flow sources)
+    kFeatAdaptive = 4,  // CFL time step
+    kFeatMap = 8,       // Manning class map
+    kFeatAll = 15,
+};
+// the instantiated feature sets, smallest first
+constexpr int kVariant0 = 0;
+constexpr int kVariant1 = kFeatMap;
+constexpr int kVariant2 = kFeatAdaptive | kFeatMap;
+constexpr int kVariant3 = kFeatStorm | kFeatAdaptive | kFeatMap;
+inline int fused_variant(int features)
+{
+    const int v[4] = {kVariant0, kVariant1, kVariant2, kVariant3};
+    for (int i = 0; i < 4; i++)
+        if ((features & ~v[i]) == 0) return v[i];
+    return kFeatAll;
+}
+inline int features_of(const StepCommon &s)
+{
+    return (s.storm_on ? kFeatStorm : 0) | (s.nst > 0 ? kFeatSources : 0) |
+           (s.fixed_dt ? 0 : kFeatAdaptive) | (s.ncls_on ? kFeatMap : 0);
+}
+
 }  // namespace sw2d
 
 struct sw2d_batch;
@@ -145,6 +172,7 @@ struct sw2d_handle {
     sw2d_params prm{};
     int device = 0;
     hipStream_t stream = nullptr;
+    int features = sw2d::kFeatAll;      // kFeat* of the current descriptor
     int esize = 8;                      // bytes per state element
     // fused-path state (ping-pong)
     void *h[2] = {nullptr, nullptr}, *um[2] = {nullptr, nullptr}, *vn[2] = {nullptr, nullptr};
@@ -210,6 +238,7 @@ struct sw2d_batch {
     int device = 0;
     int precision = SW2D_FP32;
     int esize = 4;
+    int features = 0;   // kFeat* over the attached members (OR)
     hipStream_t stream = nullptr;
     char *pool_base = nullptr;
     sw2d::Pool pool;
@@ -239,17 +268,19 @@ constexpr int kMaxMembers = 4096;
 // launchers (sw2d_fused64.hip, sw2d_fused32.hip, sw2d_plain.hip)
 // one iteration of every scenario in scen[] (tiles == NULL: the single
 // scenario scen[0], blocks in order) at launch phase `phase`
+// `features` (kFeat*) covers every scenario of the launch; the smallest
+// instantiation of the kernel that handles them runs
 hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase);
+                           int phase, int features);
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase);
+                           int phase, int features);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
 hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev);
 hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev);
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc);
-int fused_waves_per_cu_f64();  // resident one-wave blocks per CU
-int fused_waves_per_cu_f32();
+int fused_waves_per_cu_f64(int features);  // resident one-wave blocks per CU
+int fused_waves_per_cu_f32(int features);
 hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc);
 hipError_t launch_plain_init(sw2d_handle *h);
 }  // namespace sw2d

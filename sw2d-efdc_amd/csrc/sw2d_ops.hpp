@@ -88,7 +88,7 @@ bool fused(const sw2d_handle *h);
 int fused_blocks(const sw2d_handle *h);
 Scenario make_scenario(sw2d_handle *h);
 int choose_tile_rows(const sw2d_handle *h);
-long wave_slots(int device, int esize);
+long wave_slots(int device, int esize, int features);
 int blocks_at(const sw2d_handle *h, int H);
 std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &blocks);
 void set_tiles(sw2d_handle *h, int H);
