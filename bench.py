@@ -30,6 +30,7 @@ import numpy as np
 ROOT = pathlib.Path(__file__).resolve().parent
 METRIC = "cell-updates/s + % HBM roofline; p50/p99 scenario latency vs plain FP64 GPU"
 TRAFFIC_TABLE = ROOT / "profiles" / "fused_step_2048_dropin_traffic.json"
+BATCH_TRAFFIC = ROOT / "profiles" / "fused_step_batch256_traffic.json"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -88,6 +89,19 @@ def measured_traffic(precision, tile_rows, grid):
     if list(t.get("grid", [])) != list(grid):
         return None
     return t.get("bytes_per_launch", {}).get(precision, {}).get(str(tile_rows))
+
+
+def measured_batch_traffic(tile_rows, cells):
+    """HBM bytes per launch of the batch leg's launch, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same batch (None when it
+    was measured at another tile height or on another ensemble)."""
+    try:
+        t = json.load(open(BATCH_TRAFFIC))
+    except (OSError, ValueError):
+        return None
+    if t.get("tile_rows") != tile_rows or t.get("cells") != cells:
+        return None
+    return t.get("bytes_per_launch")
 
 
 def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps):
@@ -193,6 +207,9 @@ def batch_leg(pkg, cases, n, plain_rates):
         "scenarios_per_s": n / wall,
         "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_batch_traffic(H, int(sum(s.cells for s in specs))),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE of this batch; "
+                                       "profiles/fused_step_batch256_ensemble.md",
                      "bytes_per_cell_update": bytes_per_cell("fp32", False),
                      "kernel_ms_total": kms, "launches": launches},
         "max_volume_error_rel": vol_err,
