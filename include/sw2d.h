@@ -235,6 +235,18 @@ int sw2d_lake_inflow(sw2d_handle *h, double *out, int cap, int *n);
 int sw2d_timing(sw2d_handle *h, int enable);
 int sw2d_kernel_time(sw2d_handle *h, double *ms, long *launches);
 
+/* Which fused-kernel instantiation steps this handle (no counterpart in the
+ * reference: a diagnostic of this library).  *features = SW2D_FEAT_* bits the
+ * handle's scenario uses; *variant = the feature set of the instantiation
+ * that runs it, the smallest one covering *features (features outside it are
+ * compiled out).  A batch member runs the instantiation covering every member
+ * of its batch; this reports its own scenario's. */
+#define SW2D_FEAT_STORM 1
+#define SW2D_FEAT_SOURCES 2
+#define SW2D_FEAT_ADAPTIVE 4
+#define SW2D_FEAT_MAP 8
+int sw2d_kernel_features(sw2d_handle *h, int *features, int *variant);
+
 /* Device memory the handle holds, in bytes. */
 int sw2d_device_bytes(sw2d_handle *h, long long *bytes);
 

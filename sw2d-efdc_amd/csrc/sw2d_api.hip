@@ -963,6 +963,15 @@ int sw2d_kernel_time(sw2d_handle *h, double *ms, long *launches)
     return SW2D_OK;
 }
 
+int sw2d_kernel_features(sw2d_handle *h, int *features, int *variant)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    const int f = features_of(make_common(h));
+    if (features) *features = f;
+    if (variant) *variant = fused(h) ? fused_variant(f) : 0;
+    return SW2D_OK;
+}
+
 int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
 {
     if (check_idle(h) || !bytes) return SW2D_E_ARG;

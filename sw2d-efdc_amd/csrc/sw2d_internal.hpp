@@ -137,10 +137,10 @@ struct TileRef {
 // Features of a scenario that the fused kernel can compile out where no
 // scenario of a launch uses them.
 enum : int {
-    kFeatStorm = 1,     // moving storm (sw2d_set_storm)
-    kFeatSources = 2,   // point inflow series (stream_flow sources)
-    kFeatAdaptive = 4,  // CFL time step
-    kFeatMap = 8,       // Manning class map
+    kFeatStorm = SW2D_FEAT_STORM,        // moving storm (sw2d_set_storm)
+    kFeatSources = SW2D_FEAT_SOURCES,    // point inflow series (stream_flow sources)
+    kFeatAdaptive = SW2D_FEAT_ADAPTIVE,  // CFL time step
+    kFeatMap = SW2D_FEAT_MAP,            // Manning class map
     kFeatAll = 15,
 };
 // the instantiated feature sets, smallest first

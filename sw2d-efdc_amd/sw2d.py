@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(HERE, "libsw2d.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sw2d.h")
 
 FP64, FP32 = 0, 1
+# scenario features the fused kernel compiles out where unused (sw2d_kernel_features)
+FEAT_STORM, FEAT_SOURCES, FEAT_ADAPTIVE, FEAT_MAP = 1, 2, 4, 8
 FUSED, PLAIN = 0, 1
 FIELDS = {"h": 0, "um": 1, "vn": 2, "uu": 3, "vv": 4, "uua": 5, "vva": 6, "z": 7, "hm": 8,
           "hn": 9, "uu1": 10, "vv1": 11}
@@ -126,6 +128,7 @@ def lib():
         "sw2d_lake_inflow": (ct.c_int, [_H, _DP, ct.c_int, _IP]),
         "sw2d_timing": (ct.c_int, [_H, ct.c_int]),
         "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
+        "sw2d_kernel_features": (ct.c_int, [_H, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
         "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
         "sw2d_batch_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.POINTER(_H)]),
@@ -396,6 +399,15 @@ class Case:
         n = ct.c_long(0)
         self._chk(self._L.sw2d_kernel_time(self._h, ct.byref(ms), ct.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def kernel_features(self):
+        """(features, variant): SW2D_FEAT_* bits of this scenario and of the
+        fused-kernel instantiation that runs it."""
+        f = ct.c_int(0)
+        v = ct.c_int(0)
+        self._chk(self._L.sw2d_kernel_features(self._h, ct.byref(f), ct.byref(v)),
+                  "kernel_features")
+        return f.value, v.value
 
     def device_bytes(self):
         b = ct.c_longlong(0)

@@ -241,3 +241,49 @@ def test_autotune_mid_run_keeps_results(pkg, po, cases):
         assert bg[kg] == pytest.approx(bo[ko], rel=1e-12, abs=1e-9), kg
     assert abs(bg["error"]) <= 1e-9 * bg["volume"]
     assert np.array_equal(g.hydrograph()[:, 1], o.hydrograph()[:, 1])
+
+
+@pytest.mark.parametrize("feats", ["none", "map", "adaptive", "adaptive+map", "storm",
+                                   "storm+adaptive+map", "sources"])
+def test_kernel_instantiations_match_oracle(pkg, po, cases, feats):
+    """The fused kernel is compiled once per feature set (storm, point
+    sources, adaptive step, Manning map) and a launch runs the smallest
+    instantiation covering its scenario.  Every combination is equal value for
+    value to the oracle, and runs the instantiation the selection rule names."""
+    R, C = 70, 125
+    dx = 10.0
+    z = dem(cases, R, C, 21)
+    h0 = np.where(z < np.percentile(z, 25), 0.3, 0.0)
+    on = set(feats.split("+"))
+    g, o = pair(pkg, po, z, h0, dx, tile_rows=16, adaptive="adaptive" in on)
+    if "adaptive" in on:
+        o.set_adaptive(0.5, 2.0 * 0.5)
+    rain = np.array([40.0, 10.0, 0.0])
+    for c in (g, o):
+        c.set_rain(rain, 300.0)
+    if "map" in on:
+        cls = (np.arange(R * C).reshape(R, C) // 7 % 3).astype(np.uint8)
+        tab = np.array([0.02, 0.05, 0.09])
+        g.set_manning_map(cls, tab)
+        o.set_manning_map(tab[cls])
+    if "storm" in on:
+        for c in (g, o):
+            c.set_storm(150 / 3.6e6, 0.0, 0.5 * R * dx, 2.0, 0.0, 200.0, 0.0, 600.0)
+    if "sources" in on:
+        xs, ys = np.array([40 * dx + 1.0]), np.array([(R - 30) * dx + 2.0])
+        Q = np.array([[0.8], [1.2], [0.4]])
+        g.set_sources(xs, ys, Q, 200.0)
+        o.set_sources([po.source_index(R, C, 0.0, 0.0, dx, xs[0], ys[0])], Q, 200.0)
+    want = ((pkg.sw2d.FEAT_STORM if "storm" in on else 0)
+            | (pkg.sw2d.FEAT_SOURCES if "sources" in on else 0)
+            | (pkg.sw2d.FEAT_ADAPTIVE if "adaptive" in on else 0)
+            | (pkg.sw2d.FEAT_MAP if "map" in on else 0))
+    variants = [0, 8, 12, 13, 15]
+    f, v = g.kernel_features()
+    assert f == want
+    assert v == next(x for x in variants if f & ~x == 0)
+    for n in (6, 180):
+        g.step(n)
+        o.step(n)
+        compare(g, o, tag=f"{feats} after +{n}")
+    assert g.time == o.time
