@@ -33,13 +33,18 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* the three 64-entry tables of pow(x, 1.3333330) (plain data, correctly
+ * rounded values; checked entry by entry in tests/test_oracle.py) */
+#include "../sw2d-efdc_amd/csrc/sw2d_pow_tables.h"
+
 #ifdef _OPENMP
 #include <omp.h>
 #endif
 
 /* ------------------------------------------------------------------------
  * Project math: pow(x, 1.3333330) and exp(x) from correctly rounded operations
- * only (+ - * /, fma, floor, exponent bit manipulation).  The HIP library evaluates
+ * only (+ - * /, fma, floor, rint, ldexp, exponent bit manipulation, and
+ * three tables of correctly rounded constants).  The HIP library evaluates
  * the identical sequence, so both sides round identically.
  *
  * Method: argument reduction x = 2^e * m, m in (sqrt(1/2), sqrt(2)];
@@ -126,9 +131,53 @@ double orc_log(double x)
     return orc_log_pos(x);
 }
 
-/* pow(x, 1.3333330), the Manning depth exponent of SW2D-EFDC.cu:194, 311 */
+static const double ORC_POW_RECIP[64] = {SW2D_POW_RECIP};
+static const double ORC_POW_NLOG[64] = {SW2D_POW_NLOG};
+static const double ORC_POW_EXP2[64] = {SW2D_POW_EXP2};
+
+/* pow(x, 1.3333330) for 2^-700 <= x <= 2^500, table-driven without division:
+ * x = 2^e m, m in [1, 2); i = top 6 fraction bits; r = m RECIP[i] - 1 (one
+ * fma, |r| < 2^-7); ln x = e ln2 + NLOG[i] + log1p(r), log1p by Taylor to
+ * r^7; t = 1.3333330 ln x; k = rint(64 t / ln2); r' = t - k ln2/64
+ * (two-constant ln2); exp(t) = 2^(k>>6) EXP2[k&63] (1 + P(r')), P = Taylor of
+ * exp(r') - 1 to r'^6.  Horner with correctly rounded fma. */
+static double orc_pow1333_core(double x)
+{
+    uint64_t b = orc_d2bits(x);
+    int e = (int)(b >> 52) - 1023;
+    int i = (int)((b >> 46) & 63);
+    double m = orc_bits2d((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    double r = fma(m, ORC_POW_RECIP[i], -1.0);
+    double q = 1.0 / 7.0;
+    q = fma(r, q, -1.0 / 6.0);
+    q = fma(r, q, 1.0 / 5.0);
+    q = fma(r, q, -0.25);
+    q = fma(r, q, 1.0 / 3.0);
+    q = fma(r, q, -0.5);
+    double lp = fma(r * r, q, r);
+    double de = (double)e;
+    double lx = fma(de, ORC_LN2_HI, fma(de, ORC_LN2_LO, ORC_POW_NLOG[i] + lp));
+    double t = ORC_MANNING_EXP * lx;
+    double kd = rint(t * (64.0 * ORC_INV_LN2));
+    int k = (int)kd;
+    double rr = fma(-kd, ORC_LN2_HI / 64.0, t);
+    rr = fma(-kd, ORC_LN2_LO / 64.0, rr);
+    double p = 1.0 / 720.0;
+    p = fma(rr, p, 1.0 / 120.0);
+    p = fma(rr, p, 1.0 / 24.0);
+    p = fma(rr, p, 1.0 / 6.0);
+    p = fma(rr, p, 0.5);
+    double P = fma(rr * rr, p, rr);
+    double T = ORC_POW_EXP2[k & 63];
+    return ldexp(fma(T, P, T), k >> 6);
+}
+
+/* pow(x, 1.3333330), the Manning depth exponent of SW2D-EFDC.cu:194, 311:
+ * the table-driven form where nothing under- or overflows, else exp(c log x)
+ * from the general functions above */
 double orc_pow1333(double x)
 {
+    if (x >= 0x1p-700 && x <= 0x1p500) return orc_pow1333_core(x);
     if (x != x) return x;
     if (!(x > 0.0)) return x == 0.0 ? 0.0 : NAN;
     if (x == INFINITY) return INFINITY;

@@ -8,18 +8,30 @@
 // Translation units that include this header are compiled with
 // -ffp-contract=off (no fusing beyond the fma written here).
 //
-// Method: x = 2^e m, m in (sqrt(1/2), sqrt(2)]; log m = 2 atanh(s),
-// s = (m-1)/(m+1), arranged as f - (hf - s (hf + R)), hf = f^2/2,
-// R(z) = sum_k 2 z^k / (2k+1), z = s^2 (the atanh-series arrangement of Sun's
-// fdlibm e_log.c with plain Taylor coefficients); exp(t) = 2^k exp(r),
-// r = t - k ln2 with a two-constant (Cody-Waite) ln2, exp(r) by Taylor to
-// r^13; both polynomials by Horner's rule with fma.  Error against a correctly rounded pow: under 32 ulp for x in
-// [1e-5, 1e3].
+// pow for 2^-700 <= x <= 2^500 (every depth the kernels raise to the
+// power): table-driven log and exp, no division.  x = 2^e m, m in [1, 2);
+// i = the top 6 bits of m's fraction, r = m RECIP[i] - 1 (one fma, |r| <
+// 2^-7), ln x = e ln2 + NLOG[i] + log1p(r) with log1p by its Taylor series to
+// r^7; t = 1.3333330 ln x; k = rint(64 t / ln2), r' = t - k ln2/64
+// (two-constant ln2), exp(t) = 2^(k>>6) EXP2[k&63] (1 + P(r')) with P the
+// Taylor series of exp(r') - 1 to r'^6.  The tables (sw2d_pow_tables.h) hold
+// correctly rounded values.  Error against a correctly rounded pow: at most
+// 20 ulp for x in [1e-5, 1e3] (measured on 2e7 points), dominated by the
+// rounding of t.
+//
+// Other x, and exp()/log() themselves: x = 2^e m, m in (sqrt(1/2), sqrt(2)];
+// log m = 2 atanh(s), s = (m-1)/(m+1), arranged as f - (hf - s (hf + R)),
+// hf = f^2/2, R(z) = sum_k 2 z^k / (2k+1), z = s^2 (the atanh-series
+// arrangement of Sun's fdlibm e_log.c with plain Taylor coefficients);
+// exp(t) = 2^k exp(r), r = t - k ln2 with a two-constant (Cody-Waite) ln2,
+// exp(r) by Taylor to r^13; both polynomials by Horner's rule with fma.
 //
 // FP32: the hardware log2/exp2 approximations (about 1 ulp each).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "sw2d_pow_tables.h"
 
 namespace sw2d {
 
@@ -94,9 +106,46 @@ __device__ __forceinline__ double log_d(double x)
     return log_pos(x);
 }
 
+static __constant__ const double kPowRecip[64] = {SW2D_POW_RECIP};
+static __constant__ const double kPowNlog[64] = {SW2D_POW_NLOG};
+static __constant__ const double kPowExp2[64] = {SW2D_POW_EXP2};
+
+// pow(x, 1.3333330) for 2^-700 <= x <= 2^500 (table-driven, see the top)
+__device__ __forceinline__ double pow1333_core(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    const int e = (int)(b >> 52) - 1023;
+    const int i = (int)((b >> 46) & 63);
+    const double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    const double r = __builtin_fma(m, kPowRecip[i], -1.0);
+    double q = 1.0 / 7.0;
+    q = __builtin_fma(r, q, -1.0 / 6.0);
+    q = __builtin_fma(r, q, 1.0 / 5.0);
+    q = __builtin_fma(r, q, -0.25);
+    q = __builtin_fma(r, q, 1.0 / 3.0);
+    q = __builtin_fma(r, q, -0.5);
+    const double lp = __builtin_fma(r * r, q, r);
+    const double de = (double)e;
+    const double lx = __builtin_fma(de, kLn2Hi, __builtin_fma(de, kLn2Lo, kPowNlog[i] + lp));
+    const double t = kManningExp * lx;
+    const double kd = __builtin_rint(t * (64.0 * kInvLn2));
+    const int k = (int)kd;
+    double rr = __builtin_fma(-kd, kLn2Hi / 64.0, t);
+    rr = __builtin_fma(-kd, kLn2Lo / 64.0, rr);
+    double p = 1.0 / 720.0;
+    p = __builtin_fma(rr, p, 1.0 / 120.0);
+    p = __builtin_fma(rr, p, 1.0 / 24.0);
+    p = __builtin_fma(rr, p, 1.0 / 6.0);
+    p = __builtin_fma(rr, p, 0.5);
+    const double P = __builtin_fma(rr * rr, p, rr);
+    const double T = kPowExp2[k & 63];
+    return __builtin_ldexp(__builtin_fma(T, P, T), k >> 6);
+}
+
 // pow(x, 1.3333330) for FP64
 __device__ __forceinline__ double pow1333(double x)
 {
+    if (x >= 0x1p-700 && x <= 0x1p500) return pow1333_core(x);
     if (x != x) return x;
     if (!(x > 0.0)) return x == 0.0 ? 0.0 : __longlong_as_double(0x7ff8000000000000LL);
     if (x == __longlong_as_double(0x7ff0000000000000LL)) return x;

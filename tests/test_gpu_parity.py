@@ -41,6 +41,13 @@ def test_device_math_bitwise(pkg, po):
     sw = pkg.sw2d
     rng = np.random.default_rng(5)
     x = np.concatenate([10 ** rng.uniform(-5, 3, 20000), [1e-4, 1.0, 2.0, 0.5, 1e-300, 3e-320]])
+    # pow: the table-driven range [2^-700, 2^500], its edges and the general path
+    edges = [np.nextafter(b, d) for b in (2.0 ** -700, 2.0 ** 500) for d in (0, np.inf)]
+    xp = np.concatenate([x, 10 ** rng.uniform(-320, 308, 20000), edges,
+                         [2.0 ** -700, 2.0 ** 500, 0.0, -1.0, np.inf, np.nan]])
+    gp = sw.device_math(0, xp)
+    ep = po.pow1333(xp)
+    assert np.array_equal(gp, ep, equal_nan=True), "pow1333 differs from the oracle"
     got = sw.device_math(0, x)
     exp = po.pow1333(x)
     assert np.array_equal(got, exp), "pow1333 differs from the oracle"

@@ -29,6 +29,55 @@ def test_pow1333_against_libm(po):
     assert math.isnan(po.pow1333([-1.0])[0])
 
 
+def pow_tables():
+    """The three tables of sw2d_pow_tables.h (shared by the device pow and the
+    oracle's), parsed from the header's hexadecimal constants."""
+    import pathlib
+    import re
+    text = (pathlib.Path(__file__).resolve().parents[1] / "sw2d-efdc_amd" / "csrc" /
+            "sw2d_pow_tables.h").read_text()
+    out = {}
+    for name in ("SW2D_POW_RECIP", "SW2D_POW_NLOG", "SW2D_POW_EXP2"):
+        body = text.split(f"#define {name}")[1].split("\n\n")[0]
+        out[name] = np.array([float.fromhex(h) for h in re.findall(r"-?0x[0-9a-fA-Fp.+-]+", body)])
+        assert len(out[name]) == 64, name
+    return out
+
+
+def test_pow_tables():
+    """RECIP[i] = RN(1 / (1 + (i + 1/2)/64)), NLOG[i] = RN(-ln RECIP[i]),
+    EXP2[j] = RN(2^(j/64)): against IEEE division, libm's log and exp2 (within
+    an ulp), the table's own identities, and monotonicity."""
+    t = pow_tables()
+    i = np.arange(64)
+    recip, nlog, exp2 = t["SW2D_POW_RECIP"], t["SW2D_POW_NLOG"], t["SW2D_POW_EXP2"]
+    assert np.array_equal(recip, 1.0 / (1.0 + (i + 0.5) / 64.0))
+    # reciprocal of the bucket midpoint: r = m RECIP[i] - 1 stays below 2^-7
+    lo, hi = 1.0 + i / 64.0, 1.0 + (i + 1) / 64.0
+    assert np.all(np.abs(lo * recip - 1) < 2 ** -7) and np.all(np.abs(hi * recip - 1) < 2 ** -7)
+    assert ulps(nlog, -np.log(recip)).max() <= 1
+    assert np.all(np.diff(nlog) > 0) and nlog[0] > 0
+    assert ulps(exp2, np.exp2(i / 64.0)).max() <= 1
+    assert exp2[0] == 1.0 and np.all(np.diff(exp2) > 0) and exp2[-1] < 2.0
+    # 2^(j/64) * 2^((64-j)/64) = 2, to rounding
+    assert np.all(np.abs(exp2[1:] * exp2[:0:-1] - 2.0) <= 4 * np.spacing(2.0))
+
+
+def test_pow1333_paths(po):
+    """The table-driven range [2^-700, 2^500] and the general path beyond it
+    agree where they meet, and the special values hold."""
+    for b in (2.0 ** -700, 2.0 ** 500):
+        xs = np.array([np.nextafter(b, 0), b, np.nextafter(b, np.inf)])
+        got = po.pow1333(xs)
+        ref = np.power(xs, 1.3333330)
+        # |t| is large here (the rounding of t = 1.333 ln x dominates)
+        assert (np.abs(got - ref) / np.abs(ref)).max() < 1e-13
+    assert po.pow1333([1.0])[0] == 1.0
+    assert po.pow1333([np.inf])[0] == np.inf
+    assert math.isnan(po.pow1333([np.nan])[0])
+    assert po.pow1333([1e-310])[0] == 0.0
+
+
 def test_exp_log_against_libm(po):
     L = po.lib()
     rng = np.random.default_rng(2)
