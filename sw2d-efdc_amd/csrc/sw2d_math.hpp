@@ -215,14 +215,17 @@ __device__ __forceinline__ double source_value(const double *Q, int nq, int nst,
 
 // Cross-lane moves along the wavefront with DPP (GFX9 wave_shr:1 / wave_shl:1).
 // from_west: lane i receives lane i-1 (lane 0 receives 0); from_east: lane i
-// receives lane i+1 (lane 63 receives 0).  All lanes must be active.
+// receives lane i+1 (lane 63 receives 0).  All lanes must be active.  The
+// bound-control bit makes the lane without a source read 0 itself, so the
+// move needs no zeroed destination first (one instruction, foldable into the
+// instruction that consumes it).
 __device__ __forceinline__ int dpp_from_west(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int dpp_from_east(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ double from_west(double v)
 {
