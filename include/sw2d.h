@@ -304,9 +304,11 @@ int sw2d_batch_step(sw2d_batch *b, int n_iters, int *running);
 /* Progress of a member after the last sw2d_batch_step. */
 int sw2d_batch_status(sw2d_batch *b, const sw2d_handle *h, int *done, long *iters, double *time);
 
-/* Measured choice of the members' tile rows (all members must be at
- * iteration 0): each candidate runs `iters` trial launches that write only
- * spare buffers and scratch bookkeeping. */
+/* Measured choice of the members' tile rows, at any point of the run: each
+ * candidate runs `iters` trial launches of the next iteration of every
+ * running member (at the live launch phase) that write only spare buffers
+ * and scratch bookkeeping; the members are then retiled (pending per-tile
+ * partial sums are folded, so the ledger is unchanged). */
 int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows);
 
 /* Pool use in bytes: now, the most so far, capacity. */

@@ -344,9 +344,10 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     if (!b) return SW2D_E_ARG;
     if (iters < 1) return bfail(b, SW2D_E_ARG, "iters must be positive");
     if (b->members.empty()) return bfail(b, SW2D_E_STATE, "no members to tune on");
-    for (sw2d_handle *h : b->members)
-        if (h->step_index != 0) return bfail(b, SW2D_E_STATE, "autotune runs before the first iteration");
-    BCHK(b, hipStreamSynchronize(b->stream));
+    // at any point of the run: the members' progress from the device first
+    int rc = rebuild(b);
+    if (rc) return rc;
+    if ((rc = refresh(b, nullptr))) return rc;
     const int nm = (int)b->members.size();
     std::vector<int> cand = {8, 12, 16, 24, 32, 48, 64, 96, 128};
     // heights whose launch over all members fills whole rounds of wave slots
@@ -367,7 +368,6 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     std::vector<Scratch> sc(nm);
     Scenario *dscen_s = nullptr;
     TileRef *dtiles_s = nullptr;
-    int rc = SW2D_OK;
     auto cleanup = [&]() {
         for (int i = 0; i < nm; i++) {
             sw2d_handle *h = b->members[i];
@@ -399,12 +399,18 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     BCHK(b, hipEventCreate(&e1));
     int best = b->H;
     float best_ms = 1e30f;
+    // trial launches run the next iteration of every member still running,
+    // at the live launch phase: they read its clock (copied) and live buffer
+    // and write only the spare buffer and scratch bookkeeping
+    const int phase = (int)(b->phase % 6);
     for (int H : cand) {
         std::vector<Scenario> desc(nm);
         std::vector<TileRef> tiles;
         int feats = 0;
         for (int i = 0; i < nm && e == hipSuccess; i++) {
             sw2d_handle *h = b->members[i];
+            const Status &st = b->hstatus[h->member_slot];
+            if (st.done || st.fin) continue;
             const int H0 = h->H;
             set_tiles(h, h->prm.tile_rows > 0 ? h->prm.tile_rows : H);
             Scenario d = make_scenario(h);
@@ -415,7 +421,7 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
             d.s.ledger = sc[i].led;
             d.s.hydro = nullptr;
             d.s.bc_acc = sc[i].bc;
-            d.phase0 = 0;
+            d.phase0 = h->phase0;
             feats |= features_of(d.s);
             desc[i] = d;
             for (int k = 0; k < d.nblocks; k++) tiles.push_back(TileRef{i, k});
@@ -424,10 +430,10 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
         if (e == hipSuccess) e = hipMemcpy(dscen_s, desc.data(), sizeof(Scenario) * nm, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(dtiles_s, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) break;
-        // every trial launch is iteration 0 of every member again
-        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0, feats);
+        if (tiles.empty()) break;   // every member has finished
+        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
         if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), 0, feats);
+        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;
@@ -440,16 +446,10 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     cleanup();
     if (e != hipSuccess) return bhip(b, e, "batch autotune");
     b->H = best;
+    // retiling folds each member's pending partial sums into one record
     for (sw2d_handle *h : b->members) {
         if (h->prm.tile_rows > 0) continue;
-        set_tiles(h, best);
-        if (h->ntiles > h->nparts) {
-            dfree(h, h->part);
-            h->part = nullptr;
-            if ((rc = dalloc(h, &h->part, 2 * (size_t)h->ntiles))) { b->err = h->err; return rc; }
-        }
-        h->nparts = h->ntiles;
-        changed(h);
+        if ((rc = retile(h, best))) { b->err = h->err; return rc; }
     }
     b->dirty = true;
     BCHK(b, hipStreamSynchronize(b->stream));

@@ -467,6 +467,8 @@ class Batch:
         return bool(d.value), k.value, t.value
 
     def autotune(self, iters=10):
+        """Measured tile rows for the members, at any point of the run (trial
+        launches of their next iteration; the ledgers are kept)."""
         best = ct.c_int(0)
         self._chk(self._L.sw2d_batch_autotune(self._b, iters, ct.byref(best)), "batch autotune")
         return best.value

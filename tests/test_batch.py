@@ -182,3 +182,48 @@ def test_batch_autotune_keeps_results(pkg, cases):
     for i in range(3):
         assert_same(members[i], alone[i], f"autotuned member {i}")
     batch.close()
+
+
+def test_batch_autotune_mid_run_keeps_results(pkg, cases):
+    """Autotune at any point of a run: members at different iterations (one
+    joined later, so its launch phase differs), trial launches at the live
+    phase, then retiling with the pending partial sums folded.  The state
+    equals the stand-alone runs and the volume ledgers agree to rounding."""
+    batch = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
+    alone, members = [], []
+    plan = [(0, 30), (0, 41), (7, 20), (0, 26)]   # (batch iterations before attach, iterations)
+    for i, (join, n) in enumerate(plan):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision="fp32", **kw)
+        setup(a, cases, i, rain, outlet)
+        a.step(n)
+        alone.append(a)
+    for i, (join, n) in enumerate(plan):
+        if join:
+            continue
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        m = batch.case(z, h0, 10.0, **kw)
+        setup(m, cases, i, rain, outlet)
+        batch.attach(m, max_iters=n)
+        members.append((i, m))
+    batch.step(7)
+    z, h0, kw, rain, outlet = scenario(cases, 2)
+    m = batch.case(z, h0, 10.0, **kw)
+    setup(m, cases, 2, rain, outlet)
+    batch.attach(m, max_iters=plan[2][1])
+    members.append((2, m))
+    batch.step(4)                 # launch phase 11 % 6 = 5
+    H = batch.autotune(3)
+    assert 8 <= H <= 128
+    batch.step(3)
+    H2 = batch.autotune(2)        # again, at another phase, after the retiling
+    assert 8 <= H2 <= 128
+    while batch.step(6):
+        pass
+    for i, m in members:
+        assert_same(m, alone[i], f"member {i} autotuned mid-run")
+        bm, ba = m.balance(), alone[i].balance()
+        for k in ("volume", "v_source", "v_outflow"):
+            assert bm[k] == pytest.approx(ba[k], rel=1e-12, abs=1e-9), (i, k)
+        assert abs(bm["error"]) <= 1e-6 * max(bm["volume"], 1.0)
+    batch.close()
