@@ -294,3 +294,34 @@ def test_kernel_instantiations_match_oracle(pkg, po, cases, feats):
         o.step(n)
         compare(g, o, tag=f"{feats} after +{n}")
     assert g.time == o.time
+
+
+def test_grid_smaller_than_2x2_is_rejected(pkg):
+    for R, C in ((1, 1), (1, 70), (70, 1)):
+        with pytest.raises(pkg.sw2d.Sw2dError, match="at least 2x2"):
+            pkg.Case(np.zeros((R, C)), np.zeros((R, C)), 10.0, dt=0.5, manning=0.03)
+
+
+@pytest.mark.parametrize("shape", [(2, 2), (2, 70), (70, 2), (2, 3), (3, 58), (4, 59), (6, 116),
+                                   (7, 117), (17, 57), (33, 175)])
+def test_grid_shapes_match_oracle(pkg, po, cases, shape):
+    """Small and boundary shapes: two rows or two columns, widths at and
+    around the 58-column strip (one lane short, exact, one over, two strips),
+    heights around the tile height; rain onto a sloping bed with an outlet."""
+    R, C = shape
+    dx = 10.0
+    z = dem(cases, R, C, 40 + R + C)
+    h0 = np.where(z < np.median(z), 0.2, 0.0)
+    for tile_rows in (8, 0):
+        g, o = pair(pkg, po, z, h0, dx, tile_rows=tile_rows)
+        for c in (g, o):
+            c.set_rain(np.array([80.0, 20.0]), 120.0)
+            c.set_outlet(R - 1, int(np.argmin(z[R - 1])))
+        for n in (1, 5, 60):
+            g.step(n)
+            o.step(n)
+            compare(g, o, tag=f"{shape} tile {tile_rows} after +{n}")
+        assert g.time == o.time
+        bg, bo = g.balance(), o.ledger()
+        assert bg["volume"] == pytest.approx(bo["volume"], rel=1e-12, abs=1e-12)
+        g.close()
