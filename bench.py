@@ -104,13 +104,17 @@ def measured_batch_traffic(tile_rows, cells):
     return t.get("bytes_per_launch")
 
 
-def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps):
+def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps, retune=0):
     """Autotune (tile_rows 0), warm up, then time `steps` iterations with HIP
-    events on the library's stream around every launch (or graph replay)."""
+    events on the library's stream around every launch (or graph replay).
+    With retune > 0 the library re-times the tile height every `retune`
+    iterations inside the timed steps (its cost is in the wall time)."""
     case = make_case(pkg, spec, precision, kernel, tile_rows)
     case.step(warmup)
     if kernel == "fused" and tile_rows == 0:
         tile_rows = case.autotune(10)  # on the warmed-up state
+        if retune > 0:
+            case.set_retune(retune, 3)
     case.sync()
     case.timing(True)
     case.sync()
@@ -399,6 +403,8 @@ def main():
                     help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--pond-break", type=int, default=4096,
                     help="side of the pond-break precision study (FP32 form vs FP64; 0 = skip)")
+    ap.add_argument("--retune", type=int, default=1000,
+                    help="drop-in legs: re-time the tile height every N iterations (0 = once)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -417,7 +423,7 @@ def main():
 
     log(f"{R}x{C} case ready; timing {args.precision} {args.kernel}")
     case, r = timed_run(pkg, spec, args.precision, args.kernel, args.tile_rows, args.warmup,
-                        args.steps)
+                        args.steps, args.retune)
     log(f"{args.precision}: {r['value']:.3e} cell-updates/s, tile rows {r['tile_rows']}")
     value, ms_per_step, tile_rows = r["value"], r["ms_per_step"], r["tile_rows"]
     t_sim, dt2, bal, bpc, achieved, avg_s, ws_mb = (r["t_sim"], r["dt2"], r["balance"], r["bpc"],
@@ -443,6 +449,7 @@ def main():
             "kernel": args.kernel,
             "precision": args.precision,
             "tile_rows": tile_rows,
+            "retune_every": args.retune if args.kernel == "fused" else 0,
             "simulated_s_after_run": t_sim,
             "dt2_last_s": dt2,
             "device_bytes": r["device_bytes"],
@@ -467,7 +474,8 @@ def main():
     case.close()
     del case
     if args.fp32_steps > 0 and args.kernel == "fused" and args.precision == "fp64":
-        c32, r32 = timed_run(pkg, spec, "fp32", "fused", 0, args.warmup, args.fp32_steps)
+        c32, r32 = timed_run(pkg, spec, "fp32", "fused", 0, args.warmup, args.fp32_steps,
+                             args.retune)
         cmp32 = None
         if args.fp32_steps == args.steps:
             dh = c32.field("h") - h64

@@ -179,6 +179,14 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask);
  * depend on the tile height.  Synchronises. */
 int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows);
 
+/* Extension: periodic retuning.  With every > 0, sw2d_step re-times the
+ * current tile height against the heights that fill whole rounds of the
+ * device's wave slots, `iters` trial launches each, every `every`
+ * iterations, and retiles to the fastest (the wet area, and with it the
+ * best height, changes as a flood develops).  The trial launches run inside
+ * sw2d_step, which then synchronises; every = 0 turns it off. */
+int sw2d_set_retune(sw2d_handle *h, long every, int iters);
+
 /* Extension: set the fused kernel's rows per tile (1..512) at any point of a
  * run, as sw2d_autotune does with its choice.  No effect on the plain
  * kernels; the state does not depend on it. */

@@ -632,20 +632,22 @@ int sw2d_set_lake_bc(sw2d_handle *h, const int *mask)
     return SW2D_OK;
 }
 
-int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
+}  // extern "C"
+
+namespace sw2d {
+
+// the heights that fill whole rounds of wave slots on this grid
+static std::vector<int> fill_heights(const sw2d_handle *h)
 {
-    if (check_idle(h)) return SW2D_E_ARG;
-    if (iters < 1) return fail(h, SW2D_E_ARG, "iters must be positive");
-    if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tuned by sw2d_batch_autotune");
-    if (!fused(h)) {
-        if (tile_rows) *tile_rows = 0;
-        return SW2D_OK;
-    }
+    return round_fill_heights(wave_slots(h->device, h->esize, h->features),
+                              [&](int H) { return (long)blocks_at(h, H); });
+}
+
+// Times `iters` trial launches of the next iteration at each candidate
+// height and retiles to the fastest (see sw2d_autotune)
+static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *tile_rows)
+{
     HIPCHK(h, hipStreamSynchronize(h->stream));
-    std::vector<int> cand = {8, 10, 12, 14, 16, 18, 20, 24, 28, 32, 40, 48, 64, 96, 128};
-    for (int H : round_fill_heights(wave_slots(h->device, h->esize, h->features),
-                                    [&](int H) { return (long)blocks_at(h, H); }))
-        cand.push_back(H);
     std::sort(cand.begin(), cand.end());
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     // scratch bookkeeping: trials read the real state and write only the spare
@@ -705,6 +707,35 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
     return SW2D_OK;
 }
 
+}  // namespace sw2d
+
+extern "C" {
+
+int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (iters < 1) return fail(h, SW2D_E_ARG, "iters must be positive");
+    if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tuned by sw2d_batch_autotune");
+    if (!fused(h)) {
+        if (tile_rows) *tile_rows = 0;
+        return SW2D_OK;
+    }
+    std::vector<int> cand = {8, 10, 12, 14, 16, 18, 20, 24, 28, 32, 40, 48, 64, 96, 128};
+    for (int H : fill_heights(h)) cand.push_back(H);
+    return autotune_over(h, iters, cand, tile_rows);
+}
+
+int sw2d_set_retune(sw2d_handle *h, long every, int iters)
+{
+    if (check_idle(h)) return SW2D_E_ARG;
+    if (every < 0 || (every > 0 && iters < 1)) return fail(h, SW2D_E_ARG, "every >= 0 and iters >= 1");
+    if (h->attached) return fail(h, SW2D_E_STATE, "attached batch members are tuned by sw2d_batch_autotune");
+    h->retune_every = fused(h) ? every : 0;
+    h->retune_iters = iters;
+    h->next_retune = h->step_index + every;
+    return SW2D_OK;
+}
+
 int sw2d_set_tile_rows(sw2d_handle *h, int tile_rows)
 {
     if (check_idle(h)) return SW2D_E_ARG;
@@ -742,8 +773,20 @@ int sw2d_step(sw2d_handle *h, int n_iters)
     }
     int rc = upload_desc(h);
     if (rc) return rc;
-    long n = n_iters;
-    while (n > 0) {
+    long left = n_iters;
+    while (left > 0) {
+      // periodic retuning: the current height against the round-filling ones
+      if (h->retune_every > 0 && h->step_index >= h->next_retune) {
+          std::vector<int> cand = fill_heights(h);
+          cand.push_back(h->H);
+          if ((rc = autotune_over(h, h->retune_iters, cand, nullptr))) return rc;
+          if ((rc = upload_desc(h))) return rc;
+          h->next_retune = h->step_index + h->retune_every;
+      }
+      long n = left;
+      if (h->retune_every > 0) n = std::min<long>(n, std::max<long>(1, h->next_retune - h->step_index));
+      left -= n;
+      while (n > 0) {
         // whole groups of six from phase 0 replay the captured graph
         const bool group = h->graph_on && (h->step_index % 6 == 0) && n >= 6;
         if (group && (rc = ensure_graph(h))) return rc;
@@ -771,6 +814,7 @@ int sw2d_step(sw2d_handle *h, int n_iters)
         }
         advance(h, m);
         n -= m;
+      }
     }
     return SW2D_OK;
 }

@@ -173,6 +173,9 @@ struct sw2d_handle {
     int device = 0;
     hipStream_t stream = nullptr;
     int features = sw2d::kFeatAll;      // kFeat* of the current descriptor
+    long retune_every = 0;              // sw2d_set_retune: iterations between retunings (0 = off)
+    int retune_iters = 3;
+    long long next_retune = 0;
     int esize = 8;                      // bytes per state element
     // fused-path state (ping-pong)
     void *h[2] = {nullptr, nullptr}, *um[2] = {nullptr, nullptr}, *vn[2] = {nullptr, nullptr};

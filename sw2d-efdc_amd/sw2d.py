@@ -129,6 +129,7 @@ def lib():
         "sw2d_timing": (ct.c_int, [_H, ct.c_int]),
         "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
         "sw2d_kernel_features": (ct.c_int, [_H, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
+        "sw2d_set_retune": (ct.c_int, [_H, ct.c_long, ct.c_int]),
         "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
         "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
         "sw2d_batch_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.POINTER(_H)]),
@@ -399,6 +400,11 @@ class Case:
         n = ct.c_long(0)
         self._chk(self._L.sw2d_kernel_time(self._h, ct.byref(ms), ct.byref(n)), "kernel_time")
         return ms.value, n.value
+
+    def set_retune(self, every, iters=3):
+        """Re-time the tile height every `every` iterations inside step()
+        (against the round-filling heights); 0 turns it off."""
+        self._chk(self._L.sw2d_set_retune(self._h, int(every), int(iters)), "set_retune")
 
     def kernel_features(self):
         """(features, variant): SW2D_FEAT_* bits of this scenario and of the

@@ -250,6 +250,34 @@ def test_autotune_mid_run_keeps_results(pkg, po, cases):
     assert np.array_equal(g.hydrograph()[:, 1], o.hydrograph()[:, 1])
 
 
+def test_periodic_retune_keeps_results(pkg, po, cases):
+    """sw2d_set_retune: the tile height is re-timed inside step() every 25
+    iterations (at odd and even iterations, inside and across graph groups);
+    the state stays equal value for value to the oracle and the ledger and
+    hydrograph to its."""
+    spec = cases.config0(rows=90, cols=140, seed=23, outlet=True)
+    g, o = pair(pkg, po, spec.z, spec.h0, spec.cellsize, tile_rows=0, adaptive=True)
+    for c in (g, o):
+        c.set_rain(spec.rain, 600.0)
+        c.set_outlet(*spec.outlet)
+    o.set_adaptive(0.5, 1.0)
+    g.set_hydrograph(400)
+    o.set_hydrograph(400)
+    g.set_retune(25, 1)
+    g.step(31)
+    g.step(7)
+    g.step(64)
+    g.set_retune(0)
+    g.step(5)
+    o.step(107)
+    compare(g, o, tag="periodic retune")
+    bg, bo = g.balance(), o.ledger()
+    for kg, ko in (("volume", "volume"), ("v_source", "v_src"), ("v_clip", "v_clip"),
+                   ("v_outflow", "v_out"), ("q_peak", "q_peak")):
+        assert bg[kg] == pytest.approx(bo[ko], rel=1e-12, abs=1e-9), kg
+    assert np.array_equal(g.hydrograph()[:, 1], o.hydrograph()[:, 1])
+
+
 @pytest.mark.parametrize("feats", ["none", "map", "adaptive", "adaptive+map", "storm",
                                    "storm+adaptive+map", "sources"])
 def test_kernel_instantiations_match_oracle(pkg, po, cases, feats):
