@@ -91,6 +91,15 @@ def measured_traffic(precision, tile_rows, grid):
     return t.get("bytes_per_launch", {}).get(precision, {}).get(str(tile_rows))
 
 
+def measured_traffic_mix(precision, heights, grid):
+    """Launch-weighted mean of measured_traffic over (iterations, tile rows)
+    chunks (None unless the table covers every height that ran)."""
+    vals = [(n, measured_traffic(precision, H, grid)) for n, H in heights]
+    if not vals or any(t is None for _, t in vals):
+        return None
+    return int(round(sum(n * t for n, t in vals) / sum(n for n, _ in vals)))
+
+
 def measured_batch_traffic(tile_rows, cells):
     """HBM bytes per launch of the batch leg's launch, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same batch (None when it
@@ -118,8 +127,16 @@ def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps, retune=0):
     case.sync()
     case.timing(True)
     case.sync()
+    # stepped in retune-interval chunks (no extra synchronisation): a retune
+    # happens at the start of a chunk, so each chunk runs at one tile height
+    heights = []
     t0 = time.perf_counter()
-    case.step(steps)
+    left = steps
+    while left > 0:
+        n = min(left, retune) if (retune > 0 and kernel == "fused") else left
+        case.step(n)
+        heights.append((n, case.tile_rows))
+        left -= n
     case.sync()
     wall = time.perf_counter() - t0
     kms, launches = case.kernel_time()
@@ -129,7 +146,7 @@ def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps, retune=0):
     bpc = bytes_per_cell(precision, spec.nclass is not None)
     avg_s = (kms / 1e3) / max(launches, 1)
     return case, dict(value=R * C * steps / wall, ms_per_step=1e3 * wall / steps,
-                      tile_rows=tile_rows, t_sim=t_sim, dt2=dt2, balance=case.balance(), bpc=bpc,
+                      tile_rows=tile_rows, heights=heights, t_sim=t_sim, dt2=dt2, balance=case.balance(), bpc=bpc,
                       avg_s=avg_s, achieved=bpc * R * C / avg_s / 1e9,
                       ws_mb=case.device_bytes() / 1e6, device_bytes=case.device_bytes())
 
@@ -450,6 +467,7 @@ def main():
             "precision": args.precision,
             "tile_rows": tile_rows,
             "retune_every": args.retune if args.kernel == "fused" else 0,
+            "tile_rows_timed": [[n, H] for n, H in r["heights"]],
             "simulated_s_after_run": t_sim,
             "dt2_last_s": dt2,
             "device_bytes": r["device_bytes"],
@@ -460,7 +478,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(args.precision, tile_rows, [R, C]),
+            "traffic": measured_traffic_mix(args.precision, r["heights"], [R, C]),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by the 256 MiB probe "
                               "(fetch x2.00, write x1.00); profiles/fused_step_2048_dropin.md",
             "bytes_per_cell_update": bpc,
@@ -486,9 +504,11 @@ def main():
         log(f"fp32: {r32['value']:.3e} cell-updates/s, tile rows {r32['tile_rows']}")
         out["fp32_form"] = {
             "value": r32["value"], "unit": "cell-updates/s", "tile_rows": r32["tile_rows"],
+            "tile_rows_timed": [[n, H] for n, H in r32["heights"]],
             "steps": args.fp32_steps, "ms_per_step": r32["ms_per_step"],
             "roofline": {"achieved": r32["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": r32["achieved"] / HBM_PEAK_GBS,
+                         "traffic": measured_traffic_mix("fp32", r32["heights"], [R, C]),
                          "bytes_per_cell_update": r32["bpc"],
                          "kernel_avg_us": r32["avg_s"] * 1e6},
             "volume_error_rel": r32["balance"]["error"] / max(r32["balance"]["volume"], 1e-30),

@@ -187,6 +187,9 @@ int sw2d_autotune(sw2d_handle *h, int iters, int *tile_rows);
  * sw2d_step, which then synchronises; every = 0 turns it off. */
 int sw2d_set_retune(sw2d_handle *h, long every, int iters);
 
+/* The fused kernel's current tile height (rows per tile; 0 on the plain path). */
+int sw2d_get_tile_rows(sw2d_handle *h, int *rows);
+
 /* Extension: set the fused kernel's rows per tile (1..512) at any point of a
  * run, as sw2d_autotune does with its choice.  No effect on the plain
  * kernels; the state does not depend on it. */

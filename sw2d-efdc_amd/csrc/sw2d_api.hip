@@ -736,6 +736,13 @@ int sw2d_set_retune(sw2d_handle *h, long every, int iters)
     return SW2D_OK;
 }
 
+int sw2d_get_tile_rows(sw2d_handle *h, int *rows)
+{
+    if (check_idle(h) || !rows) return SW2D_E_ARG;
+    *rows = fused(h) ? h->H : 0;
+    return SW2D_OK;
+}
+
 int sw2d_set_tile_rows(sw2d_handle *h, int tile_rows)
 {
     if (check_idle(h)) return SW2D_E_ARG;

@@ -130,6 +130,7 @@ def lib():
         "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
         "sw2d_kernel_features": (ct.c_int, [_H, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         "sw2d_set_retune": (ct.c_int, [_H, ct.c_long, ct.c_int]),
+        "sw2d_get_tile_rows": (ct.c_int, [_H, _IP]),
         "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
         "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
         "sw2d_batch_create": (ct.c_int, [ct.c_int, ct.c_int, ct.c_size_t, ct.c_int, ct.POINTER(_H)]),
@@ -345,6 +346,13 @@ class Case:
 
     def set_tile_rows(self, rows):
         self._chk(self._L.sw2d_set_tile_rows(self._h, rows), "set_tile_rows")
+
+    @property
+    def tile_rows(self):
+        """The fused kernel's current tile height (0 on the plain path)."""
+        r = ct.c_int(0)
+        self._chk(self._L.sw2d_get_tile_rows(self._h, ct.byref(r)), "get_tile_rows")
+        return r.value
 
     def step(self, n=1):
         self._chk(self._L.sw2d_step(self._h, n), "step")
