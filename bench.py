@@ -101,16 +101,19 @@ def measured_traffic_mix(precision, heights, grid):
 
 
 def measured_batch_traffic(tile_rows, cells):
-    """HBM bytes per launch of the batch leg's launch, from the committed
-    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of the same batch (None when it
-    was measured at another tile height or on another ensemble)."""
+    """HBM bytes per launch of the batch leg's launch at this tile height, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE table of the same batch
+    (None for another ensemble or a height the table lacks)."""
     try:
         t = json.load(open(BATCH_TRAFFIC))
     except (OSError, ValueError):
         return None
-    if t.get("tile_rows") != tile_rows or t.get("cells") != cells:
+    if t.get("cells") != cells:
         return None
-    return t.get("bytes_per_launch")
+    per = t.get("bytes_per_launch")
+    if isinstance(per, dict):
+        return per.get(str(tile_rows))
+    return per if t.get("tile_rows") == tile_rows else None
 
 
 def timed_run(pkg, spec, precision, kernel, tile_rows, warmup, steps, retune=0):
