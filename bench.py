@@ -206,7 +206,7 @@ def batch_leg(pkg, cases, n, plain_rates):
     # initial state the flux work is skipped and the timing unrepresentative);
     # the trial launches are inside the timed region
     b.step(300)
-    H = b.autotune(5)
+    H = b.autotune(8)
     while b.step(60):
         pass
     wall = time.perf_counter() - t0

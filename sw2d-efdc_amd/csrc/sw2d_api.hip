@@ -667,10 +667,9 @@ static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *
     HIPCHK(h, hipEventCreate(&e0));
     HIPCHK(h, hipEventCreate(&e1));
     const int H0 = h->H;
-    int best = H0;
-    float best_ms = 1e30f;
-    for (int H : cand) {
-        if (H > h->R && H != cand[0]) break;
+    // mean time of n trial launches at height H (after one warm-up launch)
+    hipError_t e = hipSuccess;
+    auto trial = [&](int H, int n) -> float {
         set_tiles(h, H);
         Scenario d = make_scenario(h);
         d.s.clk = clk_s;
@@ -679,23 +678,41 @@ static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *
         d.s.ledger = led_s;
         d.s.hydro = nullptr;
         d.s.bc_acc = bc_s;
-        HIPCHK(h, hipMemcpyAsync(clk_s, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, h->stream));
-        HIPCHK(h, hipMemcpyAsync(desc_s, &d, sizeof d, hipMemcpyHostToDevice, h->stream));
-        HIPCHK(h, hipStreamSynchronize(h->stream));
+        if ((e = hipMemcpyAsync(clk_s, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, h->stream)) ||
+            (e = hipMemcpyAsync(desc_s, &d, sizeof d, hipMemcpyHostToDevice, h->stream)) ||
+            (e = hipStreamSynchronize(h->stream)))
+            return 0.0f;
         // every trial launch is the next iteration k again: it reads clock
         // slot k%3 of the scratch copy and the live buffer k%2, and writes
         // only scratch bookkeeping and the spare buffer (k+1)%2
         const int phase = (int)(h->step_index % 6);
-        hipError_t e = launch_one(h, desc_s, phase);  // warm-up of this shape
-        if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
-        HIPCHK(h, hipEventRecord(e0, h->stream));
-        for (int i = 0; i < iters; i++)
-            if ((e = launch_one(h, desc_s, phase)) != hipSuccess) return hip_fail(h, e, "autotune launch");
-        HIPCHK(h, hipEventRecord(e1, h->stream));
-        HIPCHK(h, hipEventSynchronize(e1));
+        if ((e = launch_one(h, desc_s, phase)) || (e = hipEventRecord(e0, h->stream))) return 0.0f;
+        for (int i = 0; i < n; i++)
+            if ((e = launch_one(h, desc_s, phase))) return 0.0f;
         float ms = 0;
-        HIPCHK(h, hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = H; }
+        if ((e = hipEventRecord(e1, h->stream)) || (e = hipEventSynchronize(e1)) ||
+            (e = hipEventElapsedTime(&ms, e0, e1)))
+            return 0.0f;
+        return ms / n;
+    };
+    // two stages: a short trial of every candidate, then `iters` launches of
+    // the three fastest (near-equal heights are separated by the longer trial)
+    std::vector<std::pair<float, int>> t1;
+    for (int H : cand) {
+        if (H > h->R && H != cand[0]) break;
+        const float ms = trial(H, cand.size() > 3 ? std::max(1, iters / 4) : iters);
+        if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
+        t1.emplace_back(ms, H);
+    }
+    std::sort(t1.begin(), t1.end());
+    int best = t1.front().second;
+    if (t1.size() > 3) {
+        float best_ms = 1e30f;
+        for (size_t i = 0; i < 3; i++) {
+            const float ms = trial(t1[i].second, iters);
+            if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
+            if (ms < best_ms) { best_ms = ms; best = t1[i].second; }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

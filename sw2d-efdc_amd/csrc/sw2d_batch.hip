@@ -397,13 +397,12 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     hipEvent_t e0, e1;
     BCHK(b, hipEventCreate(&e0));
     BCHK(b, hipEventCreate(&e1));
-    int best = b->H;
-    float best_ms = 1e30f;
     // trial launches run the next iteration of every member still running,
     // at the live launch phase: they read its clock (copied) and live buffer
-    // and write only the spare buffer and scratch bookkeeping
+    // and write only the spare buffer and scratch bookkeeping.  trial(H, n):
+    // mean time of n launches after one warm-up (negative when no member runs)
     const int phase = (int)(b->phase % 6);
-    for (int H : cand) {
+    auto trial = [&](int H, int n) -> float {
         std::vector<Scenario> desc(nm);
         std::vector<TileRef> tiles;
         int feats = 0;
@@ -427,19 +426,37 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
             for (int k = 0; k < d.nblocks; k++) tiles.push_back(TileRef{i, k});
             e = hipMemcpyAsync(sc[i].clk, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, b->stream);
         }
-        if (e == hipSuccess) e = hipMemcpy(dscen_s, desc.data(), sizeof(Scenario) * nm, hipMemcpyHostToDevice);
+        if (e != hipSuccess || tiles.empty()) return -1.0f;
+        e = hipMemcpy(dscen_s, desc.data(), sizeof(Scenario) * nm, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(dtiles_s, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice);
-        if (e != hipSuccess) break;
-        if (tiles.empty()) break;   // every member has finished
-        e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
+        if (e == hipSuccess) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
         if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-        for (int i = 0; i < iters && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
+        for (int i = 0; i < n && e == hipSuccess; i++) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
         if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess) break;
-        if (ms < best_ms) { best_ms = ms; best = H; }
+        return e == hipSuccess ? ms / n : -1.0f;
+    };
+    // two stages: a short trial of every candidate, then `iters` launches of
+    // the three fastest
+    int best = b->H;
+    std::vector<std::pair<float, int>> t1;
+    for (int H : cand) {
+        const float ms = trial(H, cand.size() > 3 ? std::max(1, iters / 4) : iters);
+        if (ms < 0) break;
+        t1.emplace_back(ms, H);
+    }
+    std::sort(t1.begin(), t1.end());
+    if (e == hipSuccess && !t1.empty()) {
+        best = t1.front().second;
+        if (t1.size() > 3) {
+            float best_ms = 1e30f;
+            for (size_t i = 0; i < 3 && e == hipSuccess; i++) {
+                const float ms = trial(t1[i].second, iters);
+                if (ms >= 0 && ms < best_ms) { best_ms = ms; best = t1[i].second; }
+            }
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
