@@ -140,9 +140,10 @@ int retile(sw2d_handle *h, int H)
 {
     const int old_n = h->nparts;
     const int H0 = h->H;
-    set_tiles(h, H);
-    const int new_n = h->ntiles;
+    const int new_n = ((h->C + h->W - 1) / h->W) * ((h->R + H - 1) / H);
     if (new_n == old_n && H == H0) return SW2D_OK;
+    // the new partial buffer is filled first; the handle changes only once
+    // every step has succeeded (a failure leaves tiling and buffer as they were)
     HIPCHK(h, hipStreamSynchronize(h->stream));
     Partial *np = nullptr;
     int rc = dalloc(h, &np, 2 * (size_t)new_n);
@@ -150,14 +151,19 @@ int retile(sw2d_handle *h, int H)
     if (h->step_index > 0) {
         const size_t pend = (size_t)((h->step_index + 1) % 2);
         std::vector<Partial> old((size_t)old_n);
-        HIPCHK(h, hipMemcpy(old.data(), h->part + pend * old_n, sizeof(Partial) * old_n,
-                            hipMemcpyDeviceToHost));
+        hipError_t e = hipMemcpy(old.data(), h->part + pend * old_n, sizeof(Partial) * old_n,
+                                 hipMemcpyDeviceToHost);
         Partial sum{0.0, 0.0, 0.0, 0.0};
         for (const Partial &p : old) {
             sum.vol += p.vol; sum.src += p.src; sum.clip += p.clip; sum.qout += p.qout;
         }
-        HIPCHK(h, hipMemcpy(np + pend * new_n, &sum, sizeof sum, hipMemcpyHostToDevice));
+        if (e == hipSuccess) e = hipMemcpy(np + pend * new_n, &sum, sizeof sum, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            dfree(h, np);
+            return hip_fail(h, e, "retile");
+        }
     }
+    set_tiles(h, H);
     dfree(h, h->part);
     h->part = np;
     h->nparts = new_n;
@@ -652,7 +658,9 @@ static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *
     cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
     // scratch bookkeeping: trials read the real state and write only the spare
     // buffer, a copy of the clock ring, scratch partials/ledger/inflow sums
-    const int max_tiles = ((h->C + h->W - 1) / h->W) * ((h->R + 7) / 8);
+    // (sized for the smallest height tried: the most tiles)
+    const int hmin = *std::min_element(cand.begin(), cand.end());
+    const int max_tiles = ((h->C + h->W - 1) / h->W) * ((h->R + hmin - 1) / hmin);
     Clock *clk_s = nullptr;
     Partial *part_s = nullptr;
     Ledger *led_s = nullptr;
@@ -661,14 +669,20 @@ static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *
     int rc;
     if ((rc = dalloc(h, &clk_s, 3)) || (rc = dalloc(h, &part_s, 2 * (size_t)max_tiles)) ||
         (rc = dalloc(h, &led_s, 1)) || (rc = dalloc(h, &bc_s, std::max(h->nbc, 1))) ||
-        (rc = dalloc(h, &desc_s, 1)))
+        (rc = dalloc(h, &desc_s, 1))) {
+        dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
         return rc;
-    hipEvent_t e0, e1;
-    HIPCHK(h, hipEventCreate(&e0));
-    HIPCHK(h, hipEventCreate(&e1));
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
+        return hip_fail(h, e, "hipEventCreate");
+    }
     const int H0 = h->H;
     // mean time of n trial launches at height H (after one warm-up launch)
-    hipError_t e = hipSuccess;
     auto trial = [&](int H, int n) -> float {
         set_tiles(h, H);
         Scenario d = make_scenario(h);
@@ -701,23 +715,29 @@ static int autotune_over(sw2d_handle *h, int iters, std::vector<int> cand, int *
     for (int H : cand) {
         if (H > h->R && H != cand[0]) break;
         const float ms = trial(H, cand.size() > 3 ? std::max(1, iters / 4) : iters);
-        if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
+        if (e != hipSuccess) break;
         t1.emplace_back(ms, H);
     }
     std::sort(t1.begin(), t1.end());
-    int best = t1.front().second;
-    if (t1.size() > 3) {
-        float best_ms = 1e30f;
-        for (size_t i = 0; i < 3; i++) {
-            const float ms = trial(t1[i].second, iters);
-            if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
-            if (ms < best_ms) { best_ms = ms; best = t1[i].second; }
+    int best = H0;
+    if (e == hipSuccess && !t1.empty()) {
+        best = t1.front().second;
+        if (t1.size() > 3) {
+            float best_ms = 1e30f;
+            for (size_t i = 0; i < 3 && e == hipSuccess; i++) {
+                const float ms = trial(t1[i].second, iters);
+                if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = t1[i].second; }
+            }
         }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
+    // trials change the tiling of the host mirror: back to the live one
+    // before anything else, also on failure
     set_tiles(h, H0);
+    if (e != hipSuccess) (void)hipStreamSynchronize(h->stream);
+    dfree(h, clk_s); dfree(h, part_s); dfree(h, led_s); dfree(h, bc_s); dfree(h, desc_s);
+    if (e != hipSuccess) return hip_fail(h, e, "autotune launch");
     if ((rc = retile(h, best))) return rc;
     HIPCHK(h, hipStreamSynchronize(h->stream));
     if (tile_rows) *tile_rows = best;

@@ -380,7 +380,10 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     for (int i = 0; i < nm; i++) {
         sw2d_handle *h = b->members[i];
         sc[i] = Scratch{nullptr, nullptr, nullptr, nullptr, 0};
-        sc[i].maxt = ((h->C + h->W - 1) / h->W) * ((h->R + cand[0] - 1) / cand[0]);
+        // the most tiles a trial gives this member: its own fixed height, or
+        // the smallest candidate
+        const int hmin = h->prm.tile_rows > 0 ? h->prm.tile_rows : cand[0];
+        sc[i].maxt = ((h->C + h->W - 1) / h->W) * ((h->R + hmin - 1) / hmin);
         maxblocks += sc[i].maxt + 1 + (h->nbc + 63) / 64;
     }
     for (int i = 0; i < nm && !rc; i++) {

@@ -227,3 +227,28 @@ def test_batch_autotune_mid_run_keeps_results(pkg, cases):
             assert bm[k] == pytest.approx(ba[k], rel=1e-12, abs=1e-9), (i, k)
         assert abs(bm["error"]) <= 1e-6 * max(bm["volume"], 1.0)
     batch.close()
+
+
+def test_batch_autotune_with_a_member_of_fixed_small_tiles(pkg, cases):
+    """A member created with tile_rows = 5 (below every autotune candidate)
+    keeps its tiling through a batch autotune; the trials' scratch records
+    cover its tile count.  Results equal the stand-alone runs."""
+    batch = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
+    alone, members = [], []
+    for i, rows in ((0, 5), (1, 0), (3, 0)):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision="fp32", **kw)
+        setup(a, cases, i, rain, outlet)
+        a.step(24)
+        alone.append(a)
+        m = batch.case(z, h0, 10.0, tile_rows=rows, **kw)
+        setup(m, cases, i, rain, outlet)
+        batch.attach(m, max_iters=24)
+        members.append(m)
+    batch.step(5)
+    batch.autotune(2)
+    while batch.step(6):
+        pass
+    for k, m in enumerate(members):
+        assert_same(m, alone[k], f"member {k}")
+    batch.close()

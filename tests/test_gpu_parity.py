@@ -267,9 +267,13 @@ def test_periodic_retune_keeps_results(pkg, po, cases):
     g.step(31)
     g.step(7)
     g.step(64)
+    # a height below every round-filling candidate takes part in the next
+    # retune (its scratch records are sized for it)
+    g.set_tile_rows(5)
+    g.step(30)
     g.set_retune(0)
     g.step(5)
-    o.step(107)
+    o.step(137)
     compare(g, o, tag="periodic retune")
     bg, bo = g.balance(), o.ledger()
     for kg, ko in (("volume", "volume"), ("v_source", "v_src"), ("v_clip", "v_clip"),
