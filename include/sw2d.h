@@ -401,7 +401,9 @@ int sw2d_server_release(sw2d_server *s, long id);
 int sw2d_server_wait(sw2d_server *s, long id, double timeout_s);
 int sw2d_server_stats_get(sw2d_server *s, sw2d_server_stats *out);
 const char *sw2d_server_last_error(const sw2d_server *s);
-/* Stops the device thread (queued requests are dropped) and frees everything. */
+/* Stops the device thread (queued requests are dropped) and frees everything.
+ * No other thread may be inside a sw2d_server_* call on s (a waiter must
+ * have returned first). */
 int sw2d_server_destroy(sw2d_server *s);
 
 #ifdef __cplusplus
