@@ -17,6 +17,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <condition_variable>
 #include <ctime>
 #include <deque>
@@ -245,6 +246,7 @@ int run_case(const Options &opt, const std::string &name)
 
     std::vector<double> hbuf(RC), f1(RC), f2(RC);
     AsyncWriter writer(2);
+    std::atomic<bool> write_failed{false};  // set by the writer thread
     double time = time0;
     long mstep = 0;
     int tq = -1;
@@ -285,8 +287,8 @@ int run_case(const Options &opt, const std::string &name)
             sw2d_get_field(h, SW2D_FIELD_UUA, uua_host.data());
             sw2d_get_field(h, SW2D_FIELD_VVA, vva_host.data());
             writer.submit([path = c.dir + "/EFDC/input/QSER.INP", nl = s.nlayers, snap = qser,
-                           base = s.efdc_basetime, tq] {
-                sw2dio::write_qser(path, 0, nl, snap, base, tq);
+                           base = s.efdc_basetime, tq, &write_failed] {
+                if (!sw2dio::write_qser(path, 0, nl, snap, base, tq)) write_failed = true;
             });
             if (tq == 0 && !efdc_note) {
                 printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
@@ -317,12 +319,17 @@ int run_case(const Options &opt, const std::string &name)
             vf.vv = vv.empty() ? nullptr : vv.data();
             vf.uu = uu.empty() ? nullptr : uu.data();
             vf.z = zz;
-            sw2dio::write_vtk(path, C, R, res, res, c.dem.xll, c.dem.yll, vf);
+            if (!sw2dio::write_vtk(path, C, R, res, res, c.dem.xll, c.dem.yll, vf)) write_failed = true;
         });
     }
     if (pending > 0) check(sw2d_step(h, (int)pending), "step");
     sw2d_sync(h);
     writer.finish();
+    if (write_failed) {
+        printf(" an output file could not be written (VTK or QSER.INP)\n");
+        sw2d_destroy(h);
+        return 1;
+    }
     const double duration = (std::clock() - start) / (double)CLOCKS_PER_SEC;
     printf("printf: %g\n", duration);
     FILE *ts = fopen((outdir + "TimeSimu_" + tempo + ".txt").c_str(), "w");

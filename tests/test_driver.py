@@ -177,3 +177,19 @@ def test_driver_outputs_match_oracle(cases, po, tmp_path):
     got = (pathlib.Path(base) / "EFDC" / "input" / "QSER.INP").read_text()
     assert got == qser_text(hist, CASE_KW["nlayers"], CASE_KW["basetime"])
     assert any(v > 0 for r in hist[1:] for e in ("north", "south", "east", "west") for v in r[e]["q"])
+
+
+@pytest.mark.gpu
+def test_driver_reports_an_output_write_failure(cases, tmp_path):
+    """An output file that cannot be written (here a directory stands where
+    Results_0.vtk goes) ends the run with status 1 and a message, after the
+    writer thread has drained (no silent loss of outputs)."""
+    spec = small_case(cases)
+    base = cases.write_db(spec, tmp_path, "C5", **CASE_KW)
+    outdir = pathlib.Path(base) / "SW2D" / "output"
+    outdir.mkdir(parents=True, exist_ok=True)
+    (outdir / "Results_0.vtk").mkdir()
+    r = run_driver(tmp_path, check=False)
+    assert r.returncode == 1
+    assert "could not be written" in r.stdout
+    assert (outdir / "Results_1.vtk").is_file()   # the other outputs were written
