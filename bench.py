@@ -419,6 +419,9 @@ def main():
                     help="length of the Poisson arrival window of the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-hours", type=float, default=2.0)
+    ap.add_argument("--sustained-rate", type=float, default=4.0,
+                    help="second serving session at this arrival rate (0 = skip)")
+    ap.add_argument("--sustained-seconds", type=float, default=10.0)
     ap.add_argument("--serve-active-cells", type=int, default=32 << 20,
                     help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--pond-break", type=int, default=4096,
@@ -545,6 +548,15 @@ def main():
                                    args.serve_hours, plain_rates, args.serve_active_cells)
         log(f"serving: p50 {out['serving']['p50_time_to_hydrograph_s']:.2f} s, "
             f"p99 {out['serving']['p99_time_to_hydrograph_s']:.2f} s")
+    if args.serve_seconds > 0 and args.sustained_rate > 0:
+        # the same request mix at a rate one GPU keeps up with (~half its
+        # capacity): latency without a standing queue
+        out["serving_sustained"] = serve_leg(pkg, cases, args.sustained_seconds,
+                                             args.sustained_rate, args.serve_hours, plain_rates,
+                                             args.serve_active_cells)
+        log(f"serving at {args.sustained_rate}/s: p50 "
+            f"{out['serving_sustained']['p50_time_to_hydrograph_s']:.2f} s, p99 "
+            f"{out['serving_sustained']['p99_time_to_hydrograph_s']:.2f} s")
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
