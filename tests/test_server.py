@@ -114,3 +114,37 @@ def test_active_cell_cap_orders_completion(pkg, cases):
         hq, bal = alone(pkg, z, outlet, rain, kw, t_end)
         assert np.array_equal(srv.hydrograph(ids[i]), hq)
     srv.close()
+
+
+def test_manning_map_requests_same_alone_and_in_the_mix(pkg, cases):
+    """Requests carrying a Manning class map run in a mix with requests that
+    have none (and with adaptive ones), so the batch runs the kernel instance
+    with the map; each hydrograph and ledger equals the stand-alone run's."""
+    reqs = [request(cases, i) for i in (1, 2, 3)]
+    maps = []
+    for k, (z, *_rest) in enumerate(reqs):
+        if k == 1:
+            maps.append(None)
+            continue
+        R, C = z.shape
+        cls = ((np.arange(R)[:, None] // 5 + np.arange(C)[None, :] // 7 + k) % 3).astype(np.uint8)
+        maps.append((cls, np.array([0.025, 0.05, 0.09])))
+    srv = pkg.Server(precision="fp32", pool_bytes=256 << 20, tile_rows=H, slice_iters=12)
+    ids = [srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                      hydro_cap=CAP, manning_map=mm, **kw)
+           for (z, outlet, rain, kw, t_end), mm in zip(reqs, maps)]
+    srv.wait(timeout=300)
+    for i, ((z, outlet, rain, kw, t_end), mm) in enumerate(zip(reqs, maps)):
+        r = srv.poll(ids[i])
+        assert r["state"] == "done"
+        c = pkg.Case(z, np.zeros_like(z), 10.0, precision="fp32", tile_rows=H, **kw)
+        c.set_rain(rain, 900.0)
+        c.set_outlet(*outlet)
+        if mm is not None:
+            c.set_manning_map(*mm)
+        c.set_hydrograph(CAP)
+        c.run_until(t_end)
+        assert np.array_equal(srv.hydrograph(ids[i]), c.hydrograph()), f"request {i}"
+        assert r["balance"] == c.balance(), f"request {i} ledger"
+        c.close()
+    srv.close()
