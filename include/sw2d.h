@@ -360,6 +360,7 @@ typedef struct sw2d_request {
 #define SW2D_REQ_RUNNING 1
 #define SW2D_REQ_DONE 2
 #define SW2D_REQ_FAILED (-1)
+#define SW2D_REQ_CANCELLED 3
 
 typedef struct sw2d_result {
     int state;                       /* SW2D_REQ_* */
@@ -380,6 +381,7 @@ typedef struct sw2d_server_stats {
     size_t pool_peak, pool_capacity;
     long launches;
     double kernel_ms;                /* HIP-event time of the step launches */
+    long cancelled;
 } sw2d_server_stats;
 
 /* slice_iters: iterations per batch step between admissions (0 = 60).
@@ -394,10 +396,15 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *req, long *id);
 int sw2d_server_poll(sw2d_server *s, long id, sw2d_result *out);
 /* (t, q) pairs of a finished request: 2*min(n, cap) doubles. */
 int sw2d_server_hydrograph(sw2d_server *s, long id, double *tq, long cap, long *n);
-/* Forget a finished request. */
+/* Forget a finished (done, failed or cancelled) request. */
 int sw2d_server_release(sw2d_server *s, long id);
-/* Wait until request id (id < 0: every submitted request) has finished or
- * failed; SW2D_E_STATE on timeout. */
+/* Stop a request: a queued one never runs; a running one leaves the batch
+ * after the slice in progress and its device memory returns to the pool.
+ * Its state becomes SW2D_REQ_CANCELLED (no hydrograph).  A finished request
+ * is left as it is. */
+int sw2d_server_cancel(sw2d_server *s, long id);
+/* Wait until request id (id < 0: every submitted request) has finished,
+ * failed or been cancelled; SW2D_E_STATE on timeout. */
 int sw2d_server_wait(sw2d_server *s, long id, double timeout_s);
 int sw2d_server_stats_get(sw2d_server *s, sw2d_server_stats *out);
 const char *sw2d_server_last_error(const sw2d_server *s);

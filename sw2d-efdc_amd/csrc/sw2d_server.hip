@@ -34,6 +34,7 @@ struct Request {
     sw2d_result res{};
     std::vector<double> hydro;   // (t, q) pairs after completion
     sw2d_handle *h = nullptr;    // while running
+    bool cancel = false;         // asked to stop; the device thread drops it
     size_t footprint = 0;
     long cap = 0;
     std::string err;
@@ -65,7 +66,7 @@ struct sw2d_server {
     std::vector<Request *> running;
     std::map<long, std::unique_ptr<Request>> all;
     long next_id = 0;
-    long submitted = 0, completed = 0, failed = 0;
+    long submitted = 0, completed = 0, failed = 0, cancelled = 0;
     double cell_updates = 0.0, busy = 0.0;
     // device-thread snapshots for the stats (read by other threads under mu)
     size_t pool_peak = 0, pool_cap = 0;
@@ -112,6 +113,19 @@ struct sw2d_server {
         }
         r->h = h;
         return SW2D_OK;
+    }
+
+    // device thread: a cancelled running request leaves the batch, its
+    // arrays go back to the pool, no result is kept
+    void drop(Request *r)
+    {
+        sw2d_destroy(r->h);
+        std::lock_guard<std::mutex> lk(mu);
+        r->h = nullptr;
+        active_cells -= cells_of(r);
+        r->res.state = SW2D_REQ_CANCELLED;
+        r->res.t_done = now();
+        cancelled++;
     }
 
     void retire(Request *r)
@@ -228,6 +242,15 @@ struct sw2d_server {
             }
             std::vector<Request *> keep;
             for (Request *r : running) {
+                bool cancel;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    cancel = r->cancel;
+                }
+                if (cancel) {
+                    drop(r);
+                    continue;
+                }
                 int done = 0;
                 (void)sw2d_batch_status(batch, r->h, &done, nullptr, nullptr);
                 if (done) retire(r);
@@ -354,11 +377,39 @@ int sw2d_server_release(sw2d_server *s, long id)
     auto it = s->all.find(id);
     if (it == s->all.end()) return SW2D_OK;
     const int st = it->second->res.state;
-    if (st != SW2D_REQ_DONE && st != SW2D_REQ_FAILED) {
+    if (st != SW2D_REQ_DONE && st != SW2D_REQ_FAILED && st != SW2D_REQ_CANCELLED) {
         s->err = "request has not finished";
         return SW2D_E_STATE;
     }
     s->all.erase(it);
+    return SW2D_OK;
+}
+
+int sw2d_server_cancel(sw2d_server *s, long id)
+{
+    if (!s) return SW2D_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(id);
+    if (it == s->all.end()) {
+        s->err = "unknown request id";
+        return SW2D_E_ARG;
+    }
+    Request *r = it->second.get();
+    const int st = r->res.state;
+    if (st != SW2D_REQ_QUEUED && st != SW2D_REQ_RUNNING) return SW2D_OK;  // already finished
+    auto q = std::find(s->queue.begin(), s->queue.end(), r);
+    if (q != s->queue.end()) {
+        // still queued: it never reaches the GPU
+        s->queue.erase(q);
+        r->res.state = SW2D_REQ_CANCELLED;
+        r->res.t_done = s->now();
+        s->cancelled++;
+        s->cv_done.notify_all();
+        return SW2D_OK;
+    }
+    // running, or being admitted right now: the device thread drops it after
+    // the slice in progress
+    r->cancel = true;
     return SW2D_OK;
 }
 
@@ -371,11 +422,11 @@ int sw2d_server_wait(sw2d_server *s, long id, double timeout_s)
             auto it = s->all.find(id);
             if (it == s->all.end()) return true;
             const int st = it->second->res.state;
-            return st == SW2D_REQ_DONE || st == SW2D_REQ_FAILED;
+            return st == SW2D_REQ_DONE || st == SW2D_REQ_FAILED || st == SW2D_REQ_CANCELLED;
         }
         // every submitted request has finished; the queue and the running set
         // are both empty for a moment while the device thread admits a request
-        return s->completed + s->failed == s->submitted;
+        return s->completed + s->failed + s->cancelled == s->submitted;
     };
     const bool ok = s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), finished);
     if (!ok) {
@@ -393,6 +444,7 @@ int sw2d_server_stats_get(sw2d_server *s, sw2d_server_stats *out)
     out->submitted = s->submitted;
     out->completed = s->completed;
     out->failed = s->failed;
+    out->cancelled = s->cancelled;
     out->running = (long)s->running.size();
     out->queued = (long)s->queue.size();
     out->cell_updates = s->cell_updates;

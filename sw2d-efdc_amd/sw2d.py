@@ -70,10 +70,10 @@ class ServerStats(ct.Structure):
                 ("p50_latency", ct.c_double), ("p99_latency", ct.c_double),
                 ("max_latency", ct.c_double), ("p50_wait", ct.c_double), ("p99_wait", ct.c_double),
                 ("pool_peak", ct.c_size_t), ("pool_capacity", ct.c_size_t),
-                ("launches", ct.c_long), ("kernel_ms", ct.c_double)]
+                ("launches", ct.c_long), ("kernel_ms", ct.c_double), ("cancelled", ct.c_long)]
 
 
-REQ_STATE = {0: "queued", 1: "running", 2: "done", -1: "failed"}
+REQ_STATE = {0: "queued", 1: "running", 2: "done", -1: "failed", 3: "cancelled"}
 
 _D = ct.c_double
 _DP = ct.POINTER(ct.c_double)
@@ -155,6 +155,7 @@ def lib():
         "sw2d_server_poll": (ct.c_int, [_H, ct.c_long, ct.POINTER(Result)]),
         "sw2d_server_hydrograph": (ct.c_int, [_H, ct.c_long, _DP, ct.c_long, ct.POINTER(ct.c_long)]),
         "sw2d_server_release": (ct.c_int, [_H, ct.c_long]),
+        "sw2d_server_cancel": (ct.c_int, [_H, ct.c_long]),
         "sw2d_server_wait": (ct.c_int, [_H, ct.c_long, _D]),
         "sw2d_server_stats_get": (ct.c_int, [_H, ct.POINTER(ServerStats)]),
         "sw2d_server_last_error": (ct.c_char_p, [_H]),
@@ -577,6 +578,10 @@ class Server:
 
     def release(self, rid):
         self._chk(self._L.sw2d_server_release(self._s, rid), "release")
+
+    def cancel(self, rid):
+        """Stop a queued or running request (its state becomes "cancelled")."""
+        self._chk(self._L.sw2d_server_cancel(self._s, rid), "cancel")
 
     def wait(self, rid=-1, timeout=600.0):
         self._chk(self._L.sw2d_server_wait(self._s, rid, timeout), "wait")

@@ -148,3 +148,43 @@ def test_manning_map_requests_same_alone_and_in_the_mix(pkg, cases):
         assert r["balance"] == c.balance(), f"request {i} ledger"
         c.close()
     srv.close()
+
+
+def test_cancel_queued_and_running_requests(pkg, cases):
+    """A queued request that is cancelled never runs; a running one leaves
+    the batch after its slice and its cells and memory are released, so the
+    request queued behind it is admitted and completes with the stand-alone
+    result."""
+    big = request(cases, 3)        # 200 x 200, the longest
+    small = [request(cases, i) for i in (0, 4)]
+    z, outlet, rain, kw, t_end = big
+    # the cell cap admits the big request alone
+    srv = pkg.Server(precision="fp32", pool_bytes=256 << 20, tile_rows=H, slice_iters=12,
+                     max_active_cells=z.size)
+    long_id = srv.submit(z, 10.0, t_end=20 * t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                         hydro_cap=CAP, **kw)
+    ids = [srv.submit(z2, 10.0, t_end=t2, rain=r2, dtrain=900.0, outlet=o2, hydro_cap=CAP, **k2)
+           for z2, o2, r2, k2, t2 in small]
+    deadline = time.time() + 60
+    while srv.poll(long_id)["state"] != "running" and time.time() < deadline:
+        time.sleep(0.01)
+    assert srv.poll(long_id)["state"] == "running"
+    assert srv.poll(ids[0])["state"] == "queued"
+    srv.cancel(ids[0])
+    assert srv.poll(ids[0])["state"] == "cancelled"
+    srv.cancel(long_id)
+    srv.wait(timeout=300)
+    st = srv.stats()
+    assert st["cancelled"] == 2 and st["completed"] == 1 and st["failed"] == 0
+    assert srv.poll(long_id)["state"] == "cancelled"
+    assert srv.poll(ids[0])["iterations"] == 0
+    r = srv.poll(ids[1])
+    assert r["state"] == "done"
+    z2, o2, r2, k2, t2 = small[1]
+    hq, bal = alone(pkg, z2, o2, r2, k2, t2)
+    assert np.array_equal(srv.hydrograph(ids[1]), hq)
+    assert r["balance"] == bal
+    with pytest.raises(pkg.sw2d.Sw2dError):
+        srv.hydrograph(long_id)
+    srv.release(long_id)
+    srv.close()
