@@ -391,7 +391,20 @@ def cpu_baseline(spec, seconds, threads):
             break
     return dict(value=spec.cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
                 sample=f"{n} iterations of the same {spec.z.shape[0]}x{spec.z.shape[1]} case "
-                       f"({el:.1f} s), FP64 oracle, OpenMP {threads} threads")
+                       f"({el:.1f} s), FP64 oracle, OpenMP {threads} threads",
+                cpu_model=cpu_model(), host_cpus=os.cpu_count())
+
+
+def cpu_model():
+    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -444,6 +457,10 @@ def main():
     R, C = spec.z.shape
     cells = R * C
 
+    # measured bandwidth ceiling beside the 8 TB/s vendor peak: a streaming
+    # device copy of 1 GiB (read + write), 20 times
+    copy_gbs = pkg.sw2d.stream_copy_gbs(1 << 30, 20)
+    log(f"stream copy ceiling: {copy_gbs:.0f} GB/s")
     log(f"{R}x{C} case ready; timing {args.precision} {args.kernel}")
     case, r = timed_run(pkg, spec, args.precision, args.kernel, args.tile_rows, args.warmup,
                         args.steps, args.retune)
@@ -484,6 +501,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            "stream_copy_GBs": copy_gbs,
+            "frac_of_stream_copy": achieved / copy_gbs,
             "traffic": measured_traffic_mix(args.precision, r["heights"], [R, C]),
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, scaled by the 256 MiB probe "
                               "(fetch x2.00, write x1.00); profiles/fused_step_2048_dropin.md",

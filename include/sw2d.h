@@ -269,6 +269,12 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes);
  * NULL where unused). */
 int sw2d_device_math(int device, int fn, const double *x, const double *y, double *out, int n);
 
+/* Measured bandwidth ceiling (a diagnostic with no counterpart in the
+ * reference): a streaming device-to-device copy of `bytes` bytes, 16-byte
+ * loads and stores, run `reps` times after one warm-up and timed with HIP
+ * events; *gbs = (bytes read + bytes written) / time, in GB/s. */
+int sw2d_stream_copy(int device, size_t bytes, int reps, double *gbs);
+
 /* ------------------------------------------------------------------------
  * Batches: many scenarios of any grid sizes advanced together, one kernel
  * launch per iteration for all of them (groups of six iterations replay a

@@ -1124,3 +1124,63 @@ extern "C" int sw2d_device_math(int device, int fn, const double *x, const doubl
     if (dout) (void)hipFree(dout);
     return rc;
 }
+
+namespace {
+// streaming device-to-device copy of n 16-byte elements: each thread moves
+// four elements a block-width apart per step (four 16-byte loads in flight
+// before the stores; stores bypass the caches), grid-stride over the rest
+__global__ void __launch_bounds__(256) stream_copy(const double2 *__restrict__ in,
+                                                   double2 *__restrict__ out, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x * 4;
+    size_t i = (size_t)blockIdx.x * blockDim.x * 4 + threadIdx.x;
+    for (; i + 3 * blockDim.x < n; i += stride) {
+        const double2 a = in[i], b = in[i + blockDim.x], c = in[i + 2 * blockDim.x],
+                      d = in[i + 3 * blockDim.x];
+        __builtin_nontemporal_store(a.x, &out[i].x); __builtin_nontemporal_store(a.y, &out[i].y);
+        __builtin_nontemporal_store(b.x, &out[i + blockDim.x].x);
+        __builtin_nontemporal_store(b.y, &out[i + blockDim.x].y);
+        __builtin_nontemporal_store(c.x, &out[i + 2 * blockDim.x].x);
+        __builtin_nontemporal_store(c.y, &out[i + 2 * blockDim.x].y);
+        __builtin_nontemporal_store(d.x, &out[i + 3 * blockDim.x].x);
+        __builtin_nontemporal_store(d.y, &out[i + 3 * blockDim.x].y);
+    }
+    for (; i < n; i += blockDim.x) out[i] = in[i];
+}
+}  // namespace
+
+extern "C" int sw2d_stream_copy(int device, size_t bytes, int reps, double *gbs)
+{
+    if (!gbs || reps < 1 || bytes < 4096) return SW2D_E_ARG;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return SW2D_E_NODEV;
+    if (hipSetDevice(device) != hipSuccess) return SW2D_E_HIP;
+    const size_t n = bytes / sizeof(double2);
+    double2 *in = nullptr, *out = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = SW2D_OK;
+    if (hipMalloc(&in, n * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&out, n * sizeof(double2)) != hipSuccess)
+        rc = SW2D_E_NOMEM;
+    if (!rc && (hipMemset(in, 0, n * sizeof(double2)) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = SW2D_E_HIP;
+    float ms = 0;
+    if (!rc) {
+        const unsigned blocks = (unsigned)std::min<size_t>((n + 1023) / 1024, 16384);
+        hipLaunchKernelGGL(stream_copy, dim3(blocks), dim3(256), 0, 0, in, out, n);  // warm-up
+        (void)hipEventRecord(e0, 0);
+        for (int r = 0; r < reps; r++)
+            hipLaunchKernelGGL(stream_copy, dim3(blocks), dim3(256), 0, 0, in, out, n);
+        (void)hipEventRecord(e1, 0);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0)
+            rc = SW2D_E_HIP;
+    }
+    if (!rc) *gbs = 2.0 * (double)(n * sizeof(double2)) * reps / (ms * 1e-3) / 1e9;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (in) (void)hipFree(in);
+    if (out) (void)hipFree(out);
+    return rc;
+}

@@ -130,6 +130,7 @@ def lib():
         "sw2d_kernel_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long)]),
         "sw2d_kernel_features": (ct.c_int, [_H, ct.POINTER(ct.c_int), ct.POINTER(ct.c_int)]),
         "sw2d_set_retune": (ct.c_int, [_H, ct.c_long, ct.c_int]),
+        "sw2d_stream_copy": (ct.c_int, [ct.c_int, ct.c_size_t, ct.c_int, _DP]),
         "sw2d_get_tile_rows": (ct.c_int, [_H, _IP]),
         "sw2d_device_bytes": (ct.c_int, [_H, ct.POINTER(ct.c_longlong)]),
         "sw2d_device_math": (ct.c_int, [ct.c_int, ct.c_int, _DP, _DP, _DP, ct.c_int]),
@@ -179,6 +180,15 @@ def _ip(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=20, device=0):
+    """Measured HBM ceiling: GB/s (read + write) of a streaming device copy."""
+    g = ct.c_double(0)
+    rc = lib().sw2d_stream_copy(device, nbytes, reps, ct.byref(g))
+    if rc:
+        raise Sw2dError(f"sw2d_stream_copy failed ({rc})")
+    return g.value
 
 
 def device_math(fn, x, y=None, device=0):
