@@ -313,6 +313,54 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0):
     }
 
 
+def config0_leg(pkg, cases, threads):
+    """The reference's own CPU-runnable case (256x256, dx 10 m, 1 h of rain and
+    recession, fixed dt 0.5 s, 3600 iterations), the whole run: the FP64 fused
+    kernel (graph replays) against the FP64 oracle on the host cores, and the
+    final states compared value for value."""
+    spec = cases.config0(outlet=True)
+    R, C = spec.z.shape
+    iters = int(round(spec.t_end / (2 * spec.dt)))
+    c = pkg.Case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+    c.set_rain(spec.rain, spec.dtrain)
+    c.set_outlet(*spec.outlet)
+    c.step(6)
+    c.sync()
+    c.close()   # the first case paid the one-off setup (graph capture, module load)
+    c = pkg.Case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+    c.set_rain(spec.rain, spec.dtrain)
+    c.set_outlet(*spec.outlet)
+    c.sync()
+    t0 = time.perf_counter()
+    c.step(iters)
+    c.sync()
+    gpu_s = time.perf_counter() - t0
+    h_gpu = c.field("h")
+    c.close()
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as po
+    o = po.Oracle(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning,
+                  threads=threads)
+    o.set_rain(spec.rain, spec.dtrain)
+    o.set_outlet(*spec.outlet)
+    t0 = time.perf_counter()
+    o.step(iters)
+    cpu_s = time.perf_counter() - t0
+    same = bool(np.array_equal(h_gpu, o.field("h")))
+    return {
+        "workload": "config 0: 256x256, dx 10 m, 1 % slope + fBm 2 m, n 0.03, rain 50 mm/h for "
+                    "30 min then 30 min recession, fixed dt 0.5 s, outlet weir; 3600 iterations",
+        "iterations": iters,
+        "gpu_fp64_s": gpu_s,
+        "gpu_cell_updates_per_s": R * C * iters / gpu_s,
+        "cpu_oracle_s": cpu_s,
+        "cpu_threads": threads,
+        "cpu_cell_updates_per_s": R * C * iters / cpu_s,
+        "speedup": cpu_s / gpu_s,
+        "final_h_equal": same,
+    }
+
+
 def pond_break_leg(pkg, cases, side, sample=30.0):
     """The pond-break case (square reservoir 4 m deep released onto a dry
     fBm floodplain, dx 2 m, n 0.035, adaptive step, 30 min) in the FP32 form
@@ -444,6 +492,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-config0", dest="config0", action="store_false",
+                    help="skip the config-0 leg (whole 256x256 run, GPU vs CPU oracle)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -577,6 +627,12 @@ def main():
             f"{out['serving_sustained']['p50_time_to_hydrograph_s']:.2f} s, p99 "
             f"{out['serving_sustained']['p99_time_to_hydrograph_s']:.2f} s")
 
+    if not args.no_cpu and args.config0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        out["config0"] = config0_leg(pkg, cases, threads)
+        log(f"config 0: GPU {out['config0']['gpu_fp64_s']:.3f} s, CPU oracle "
+            f"{out['config0']['cpu_oracle_s']:.2f} s ({threads} threads), final h equal: "
+            f"{out['config0']['final_h_equal']}")
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads, {args.cpu_seconds} s")

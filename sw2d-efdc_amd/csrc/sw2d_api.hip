@@ -203,16 +203,19 @@ int blocks_at(const sw2d_handle *h, int H)
     return nstrips * ((h->R + H - 1) / H) + 1 + (h->nbc + 63) / 64;
 }
 
-// Tile heights in [8, 128] that fill whole rounds of wave slots: for k = 1..6
+// Tile heights in [2, 128] that fill whole rounds of wave slots: for k = 1..6
 // the smallest H whose launch fits in k rounds.  A launch slightly over a
 // whole number of rounds leaves a nearly empty last round (on a 2048^2 grid
 // in FP64, 2016 tiles at H = 37 run 12% faster than 2 rounds at H = 20 and
 // 17% faster than 2.25 rounds at H = 16), and within a round a larger H
-// re-reads fewer halo rows per output row.
+// re-reads fewer halo rows per output row.  On grids too small to fill one
+// round the launch time is the tile's row chain (H + 6 rows in sequence), so
+// the smallest height that still fits one round is the fastest (256^2: 16.7
+// us per iteration at H = 2 against 28.2 at H = 8).
 std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &blocks)
 {
     std::vector<int> out;
-    int H = 8;
+    int H = 2;
     for (int k = 6; k >= 1; k--) {   // the smallest height grows as k falls
         while (H <= 128 && blocks(H) > k * slots) H++;
         if (H > 128) break;
