@@ -247,13 +247,12 @@ def batch_leg(pkg, cases, n, plain_rates):
     }
 
 
-def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0):
+def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0, tile=0):
     """A bounded serving session: Poisson arrivals at `rate`/s for `seconds`,
     grid sides 256..4096 (p 0.4/0.3/0.15/0.1/0.05), 2 h storms, FP32 form."""
     n = max(1, int(round(seconds * rate)))
     arrivals, specs = cases.serve_requests(n, rate=rate, t_end=3600.0 * hours)
     log(f"serving: {n} requests generated; submitting")
-    tile = 32
     srv = pkg.Server(precision="fp32", pool_bytes=64 << 30, tile_rows=tile, slice_iters=60,
                      max_active_cells=active_cells)
     cap = int(3600.0 * hours / (2 * 0.5)) + 16
@@ -302,6 +301,8 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0):
         "p50_queue_s": st["p50_wait"],
         "p99_queue_s": st["p99_wait"],
         "pool_peak_bytes": st["pool_peak"],
+        "tile_rows": tile if tile > 0 else "adaptive",
+        "retiles": st["retiles"],
         "roofline": {"achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
                      "bytes_per_cell_update": bytes_per_cell("fp32", False)},
@@ -483,6 +484,8 @@ def main():
     ap.add_argument("--sustained-rate", type=float, default=4.0,
                     help="second serving session at this arrival rate (0 = skip)")
     ap.add_argument("--sustained-seconds", type=float, default=10.0)
+    ap.add_argument("--serve-tile-rows", type=int, default=0,
+                    help="server tile height (0 = adaptive: retiled to the running set)")
     ap.add_argument("--serve-active-cells", type=int, default=32 << 20,
                     help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--pond-break", type=int, default=4096,
@@ -614,7 +617,8 @@ def main():
         log(f"batch: {out['batch']['value']:.3e} cell-updates/s")
     if args.serve_seconds > 0:
         out["serving"] = serve_leg(pkg, cases, args.serve_seconds, args.serve_rate,
-                                   args.serve_hours, plain_rates, args.serve_active_cells)
+                                   args.serve_hours, plain_rates, args.serve_active_cells,
+                                   args.serve_tile_rows)
         log(f"serving: p50 {out['serving']['p50_time_to_hydrograph_s']:.2f} s, "
             f"p99 {out['serving']['p99_time_to_hydrograph_s']:.2f} s")
     if args.serve_seconds > 0 and args.sustained_rate > 0:
@@ -622,7 +626,7 @@ def main():
         # capacity): latency without a standing queue
         out["serving_sustained"] = serve_leg(pkg, cases, args.sustained_seconds,
                                              args.sustained_rate, args.serve_hours, plain_rates,
-                                             args.serve_active_cells)
+                                             args.serve_active_cells, args.serve_tile_rows)
         log(f"serving at {args.sustained_rate}/s: p50 "
             f"{out['serving_sustained']['p50_time_to_hydrograph_s']:.2f} s, p99 "
             f"{out['serving_sustained']['p99_time_to_hydrograph_s']:.2f} s")

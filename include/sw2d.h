@@ -388,9 +388,16 @@ typedef struct sw2d_server_stats {
     long launches;
     double kernel_ms;                /* HIP-event time of the step launches */
     long cancelled;
+    long tile_rows;                  /* tile height new members start at */
+    long retiles;                    /* members moved to another height (adaptive tiling) */
 } sw2d_server_stats;
 
-/* slice_iters: iterations per batch step between admissions (0 = 60).
+/* tile_rows: tile height of every request that names none (0 = adaptive:
+ * after each admission or retirement the running requests are retiled to the
+ * height that fills whole rounds of wave slots, the fewest rounds with a
+ * height of at most 48, else 32; results are those of any fixed height, the
+ * volume ledger to rounding).
+ * slice_iters: iterations per batch step between admissions (0 = 60).
  * max_active_cells: admission also stops while the running scenarios hold
  * this many cells (0 = no limit; a request is always admitted onto an idle
  * GPU).  With enough cells in flight to fill the GPU, admitting more only

@@ -150,6 +150,52 @@ size_t pool_bytes_of(size_t bytes) { return Pool::rounded(bytes); }
 
 }  // namespace
 
+namespace sw2d {
+
+// Round-filling tile height for the members still running (a member with a
+// height of its own keeps it): the fewest whole rounds of wave slots whose
+// height is at most 48, else 32 when even six rounds need taller tiles.  The
+// running members are retiled to it and new members start at it.  A member
+// whose retile finds no pool memory keeps its height (members of one launch
+// may differ in height).  *retiled counts the members moved.
+int batch_fit_tiles(sw2d_batch *b, int *tile_rows, int *retiled)
+{
+    if (retiled) *retiled = 0;
+    int rc = refresh(b, nullptr);
+    if (rc) return rc;
+    std::vector<sw2d_handle *> run;
+    int feats = 0;
+    for (sw2d_handle *h : b->members) {
+        const Status &st = b->hstatus[h->member_slot];
+        if (st.done || st.fin) continue;
+        run.push_back(h);
+        feats |= features_of(make_common(h));
+    }
+    if (!run.empty()) {
+        const long slots = wave_slots(b->device, b->esize, feats);
+        const std::vector<int> fit = round_fill_heights(slots, [&](int H) {
+            long n = 0;
+            for (const sw2d_handle *h : run) n += blocks_at(h, h->prm.tile_rows > 0 ? h->prm.tile_rows : H);
+            return n;
+        });
+        int best = 32;
+        for (auto it = fit.rbegin(); it != fit.rend(); ++it)
+            if (*it <= 48) { best = *it; break; }
+        b->H = best;
+        for (sw2d_handle *h : run) {
+            if (h->prm.tile_rows > 0 || h->H == best) continue;
+            rc = retile(h, best);
+            if (rc == SW2D_E_NOMEM) continue;
+            if (rc) { b->err = h->err; return rc; }
+            if (retiled) (*retiled)++;
+        }
+    }
+    if (tile_rows) *tile_rows = b->H;
+    return SW2D_OK;
+}
+
+}  // namespace sw2d
+
 extern "C" {
 
 int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_rows, sw2d_batch **out)

@@ -7,7 +7,11 @@
 // pool, and (2) admits queued requests in arrival order while the pool can
 // hold them.  Everything admitted is stepped by the same launches, so a
 // request that arrives while others run starts at the next slice boundary
-// instead of waiting for the GPU to drain.
+// instead of waiting for the GPU to drain.  With adaptive tiling the running
+// set is retiled after each change of membership to the height that fills
+// whole rounds of wave slots: a few small requests run as many short tiles
+// (their time per iteration is the tile's row chain), a full GPU as fewer,
+// taller ones (fewer halo rows re-read).
 #include <math.h>
 
 #include <algorithm>
@@ -55,6 +59,10 @@ double percentile(std::vector<double> v, double p)
 
 struct sw2d_server {
     int device = 0, precision = SW2D_FP32, tile_rows = 32, slice = 60;
+    bool adaptive = false;     // tile height follows the running set
+    bool refit = false;        // membership changed since the last fit
+    int cur_tile = 32;
+    long retiles = 0;
     long max_active = 0;       // cell budget of the running set (0 = none)
     long active_cells = 0;     // cells of the running scenarios
     sw2d_batch *batch = nullptr;
@@ -120,6 +128,7 @@ struct sw2d_server {
     void drop(Request *r)
     {
         sw2d_destroy(r->h);
+        refit = true;
         std::lock_guard<std::mutex> lk(mu);
         r->h = nullptr;
         active_cells -= cells_of(r);
@@ -143,6 +152,7 @@ struct sw2d_server {
         const double cells = (double)r->req.grid.rows * (double)r->req.grid.cols;
         std::string e = rc ? sw2d_last_error(h) : "";
         sw2d_destroy(h);
+        refit = true;
         std::lock_guard<std::mutex> lk(mu);
         r->h = nullptr;
         active_cells -= cells_of(r);
@@ -204,6 +214,17 @@ struct sw2d_server {
                 r->res.t_admit = now();
                 active_cells += cells_of(r);
                 running.push_back(r);
+                refit = true;
+            }
+            if (adaptive && refit && !running.empty()) {
+                // a failed retile leaves that member at its height; a HIP
+                // error shows again at the next step
+                int H = cur_tile, n = 0;
+                (void)batch_fit_tiles(batch, &H, &n);
+                refit = false;
+                std::lock_guard<std::mutex> lk(mu);
+                cur_tile = H;
+                retiles += n;
             }
             if (running.empty()) {
                 std::unique_lock<std::mutex> lk(mu);
@@ -277,7 +298,9 @@ int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_ro
     sw2d_server *s = new sw2d_server();
     s->device = device;
     s->precision = precision;
+    s->adaptive = tile_rows == 0;
     s->tile_rows = tile_rows > 0 ? tile_rows : 32;
+    s->cur_tile = s->tile_rows;
     s->slice = slice_iters > 0 ? slice_iters : 60;
     s->max_active = max_active_cells > 0 ? max_active_cells : 0;
     int rc = sw2d_batch_create(device, precision, pool_bytes, s->tile_rows, &s->batch);
@@ -328,7 +351,8 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
     r->cap = std::max<long>(cap, 1);
     sw2d_params p = q->params;
     p.precision = s->precision;
-    r->footprint = sw2d_batch_footprint(&q->grid, &p, s->tile_rows, r->cap, q->nclass > 0);
+    // adaptive tiling may take a member down to 2-row tiles
+    r->footprint = sw2d_batch_footprint(&q->grid, &p, s->adaptive ? 2 : s->tile_rows, r->cap, q->nclass > 0);
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;
     r->res.state = SW2D_REQ_QUEUED;
@@ -459,6 +483,8 @@ int sw2d_server_stats_get(sw2d_server *s, sw2d_server_stats *out)
     out->pool_capacity = s->pool_cap;
     out->launches = s->launches;
     out->kernel_ms = s->kernel_ms;
+    out->tile_rows = s->cur_tile;
+    out->retiles = s->retiles;
     return SW2D_OK;
 }
 

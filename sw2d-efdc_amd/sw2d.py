@@ -70,7 +70,8 @@ class ServerStats(ct.Structure):
                 ("p50_latency", ct.c_double), ("p99_latency", ct.c_double),
                 ("max_latency", ct.c_double), ("p50_wait", ct.c_double), ("p99_wait", ct.c_double),
                 ("pool_peak", ct.c_size_t), ("pool_capacity", ct.c_size_t),
-                ("launches", ct.c_long), ("kernel_ms", ct.c_double), ("cancelled", ct.c_long)]
+                ("launches", ct.c_long), ("kernel_ms", ct.c_double), ("cancelled", ct.c_long),
+                ("tile_rows", ct.c_long), ("retiles", ct.c_long)]
 
 
 REQ_STATE = {0: "queued", 1: "running", 2: "done", -1: "failed", 3: "cancelled"}
@@ -516,9 +517,11 @@ class Batch:
 
 
 class Server:
-    """Continuous-batching request loop on a device thread (sw2d_server_*)."""
+    """Continuous-batching request loop on a device thread (sw2d_server_*).
+    tile_rows=0 retiles the running requests after each admission or
+    retirement to the height that fills whole rounds of wave slots."""
 
-    def __init__(self, precision="fp32", pool_bytes=8 << 30, tile_rows=32, slice_iters=60,
+    def __init__(self, precision="fp32", pool_bytes=8 << 30, tile_rows=0, slice_iters=60,
                  max_active_cells=0, device=0):
         self._L = lib()
         self._s = _H()

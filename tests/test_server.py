@@ -188,3 +188,41 @@ def test_cancel_queued_and_running_requests(pkg, cases):
         srv.hydrograph(long_id)
     srv.release(long_id)
     srv.close()
+
+
+def test_adaptive_tiling_keeps_results(pkg, cases):
+    """tile_rows=0: the server retiles the running requests after admissions
+    and retirements (grids near one round of wave slots at 2-row tiles, at
+    most three running, so the height moves as the set changes while
+    requests are mid-run).  Hydrographs equal the stand-alone runs value for
+    value (the outlet is one cell of the state, which no tiling changes);
+    the volume ledger, whose pending partial sums a retile folds, to
+    rounding."""
+    shapes = [(580, 580), (560, 600), (600, 540), (520, 640), (590, 570)]
+    reqs = []
+    for i, (R, C) in enumerate(shapes):
+        rng = np.random.default_rng(900 + i)
+        r = np.arange(R, dtype=np.float64)[:, None]
+        z = 0.01 * 10.0 * (R - 1 - r) + cases.fbm(R, C, 0.7, rng.uniform(0.5, 3.0), 900 + i)
+        z = np.broadcast_to(z, (R, C)).copy()
+        outlet = (R - 1, int(np.argmin(z[R - 1])))
+        rain = np.array([rng.uniform(20, 90) * 0.25, 0.0])
+        kw = dict(dt=0.5, manning=rng.uniform(0.03, 0.08), adaptive=bool(i == 3))
+        reqs.append((z, outlet, rain, kw, 150.0 + 60.0 * ((i * 3) % 5)))
+    cap = int(3.2 * max(z.size for z, *_ in reqs))
+    srv = pkg.Server(precision="fp32", pool_bytes=1 << 30, tile_rows=0, slice_iters=12,
+                     max_active_cells=cap)
+    ids = [srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                      hydro_cap=CAP, **kw) for z, outlet, rain, kw, t_end in reqs]
+    srv.wait(timeout=300)
+    st = srv.stats()
+    assert st["completed"] == len(reqs) and st["failed"] == 0
+    assert st["retiles"] >= 1 and 2 <= st["tile_rows"] <= 48
+    for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
+        r = srv.poll(ids[i])
+        assert r["state"] == "done"
+        hq, bal = alone(pkg, z, outlet, rain, kw, t_end)
+        assert np.array_equal(srv.hydrograph(ids[i]), hq), f"request {i} hydrograph"
+        for k in ("volume", "v_source", "v_clip", "v_outflow"):
+            assert r["balance"][k] == pytest.approx(bal[k], rel=1e-12, abs=1e-9), (i, k)
+    srv.close()
