@@ -362,6 +362,65 @@ def config0_leg(pkg, cases, threads):
     }
 
 
+def output_leg(pkg, spec, iters, every, precision="fp64"):
+    """The output path: every `every` iterations the five fields the driver
+    writes (depth, the two cell-centre velocities, the two coupling
+    velocities), taken (a) with blocking copies into host arrays
+    (sw2d_get_field, the reference's output block) and (b) as double-buffered
+    snapshots that a writer thread waits for (views of the pinned buffers)
+    and releases while the run goes on.  Wall time of `iters` iterations each
+    way beside the same iterations with no output: the cost of an output to
+    the stepping thread, file formatting not included (the driver does that
+    on its writer thread in both cases).  Snapshot buffers and the derive
+    scratch are made before the timed part."""
+    import threading
+    import queue
+    fields = ("h", "uu", "vv", "uua", "vva")
+    R, C = spec.z.shape
+    wall = {}
+    for mode in ("none", "blocking", "snapshot"):
+        c = make_case(pkg, spec, precision, "fused", 0)
+        c.step(every)
+        if mode == "blocking":
+            bufs = {n: np.zeros((R, C)) for n in fields}   # allocated once, as the driver does
+            c.field("h", bufs["h"])
+        q, th = queue.Queue(), None
+        if mode == "snapshot":
+            ts = [c.snapshot(fields), c.snapshot(fields)]
+            for t in ts:
+                c.snapshot_release(t)
+
+            def writer():
+                while (t := q.get()) is not None:
+                    for n in fields:
+                        c.snapshot_field(t, n, copy=False)
+                    c.snapshot_release(t)
+            th = threading.Thread(target=writer)
+            th.start()
+        c.sync()
+        t0 = time.perf_counter()
+        for _ in range(iters // every):
+            c.step(every)
+            if mode == "blocking":
+                for n in fields:
+                    c.field(n, bufs[n])
+            elif mode == "snapshot":
+                q.put(c.snapshot(fields, timeout=600.0))
+        if th is not None:
+            q.put(None)
+            th.join()
+        c.sync()
+        wall[mode] = time.perf_counter() - t0
+        c.close()
+    n_out = iters // every
+    return {
+        "workload": f"{R}x{C} {precision} fused, {iters} iterations, an output every {every} "
+                    f"({n_out} outputs of {len(fields)} fields, {len(fields) * R * C * 8 / 1e6:.0f} MB each)",
+        "wall_s": wall,
+        "output_cost_ms": {m: 1e3 * (wall[m] - wall["none"]) / n_out for m in ("blocking", "snapshot")},
+    }
+
+
 def pond_break_leg(pkg, cases, side, sample=30.0):
     """The pond-break case (square reservoir 4 m deep released onto a dry
     fBm floodplain, dx 2 m, n 0.035, adaptive step, 30 min) in the FP32 form
@@ -490,6 +549,9 @@ def main():
                     help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--pond-break", type=int, default=4096,
                     help="side of the pond-break precision study (FP32 form vs FP64; 0 = skip)")
+    ap.add_argument("--output-every", type=int, default=100,
+                    help="output-path leg: iterations between outputs (0 = skip the leg)")
+    ap.add_argument("--output-iters", type=int, default=1000)
     ap.add_argument("--retune", type=int, default=1000,
                     help="drop-in legs: re-time the tile height every N iterations (0 = once)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -606,6 +668,11 @@ def main():
                              "steps": args.plain_steps,
                              "fused_speedup": value / pv}
         del plain
+
+    if args.output_every > 0:
+        out["output_path"] = output_leg(pkg, spec, args.output_iters, args.output_every)
+        oc = out["output_path"]["output_cost_ms"]
+        log(f"outputs: {oc['blocking']:.1f} ms each blocking, {oc['snapshot']:.1f} ms as snapshots")
 
     if args.pond_break > 0:
         out["precision_study"] = pond_break_leg(pkg, cases, args.pond_break)

@@ -217,6 +217,21 @@ int sw2d_time(sw2d_handle *h, double *time, long *mstep, double *dt2_next);
  * 5042, 5058).  Synchronises. */
 int sw2d_get_field(sw2d_handle *h, int field, double *out);
 
+/* Asynchronous output copies (double-buffered; stand-alone handles).  Same
+ * values as sw2d_get_field, but the blocking copies at output times (4880-4916,
+ * 4976-5072) become a snapshot that overlaps the steps that follow:
+ * sw2d_snapshot queues, in stream order, the fields of `fields` (bit
+ * 1u << SW2D_FIELD_x) at the current step and their copy into pinned host
+ * memory on a copy stream, and returns without waiting.  Two snapshots may be
+ * held at once; a third waits up to timeout_s for a release, then fails with
+ * SW2D_E_STATE.  sw2d_snapshot_field waits for the copy and gives rows*cols
+ * doubles valid until sw2d_snapshot_release.  sw2d_snapshot_field and
+ * sw2d_snapshot_release may be called from another thread (a writer) while
+ * this one steps the handle. */
+int sw2d_snapshot(sw2d_handle *h, unsigned fields, double timeout_s, long *ticket);
+int sw2d_snapshot_field(sw2d_handle *h, long ticket, int field, const double **data);
+int sw2d_snapshot_release(sw2d_handle *h, long ticket);
+
 /* couple_BC_efdc (728-1067) on caller-supplied host fields: records of the
  * four edge lists (S, N, E, W in the reference's naming) with the EFDC
  * (I, J) and inflow (m3/s).  Arrays cells/I/J/q hold 4*cap entries, list k

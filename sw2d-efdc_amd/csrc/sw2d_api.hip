@@ -450,6 +450,7 @@ int sw2d_destroy(sw2d_handle *h)
         if (it != own.end()) own.erase(it);
     }
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    snapshot_free(h);
     for (auto &ev : h->events) {
         (void)hipEventDestroy(ev.first);
         (void)hipEventDestroy(ev.second);

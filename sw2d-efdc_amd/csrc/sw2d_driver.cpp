@@ -244,9 +244,16 @@ int run_case(const Options &opt, const std::string &name)
     }
     bool efdc_note = false;
 
-    std::vector<double> hbuf(RC), f1(RC), f2(RC);
     AsyncWriter writer(2);
     std::atomic<bool> write_failed{false};  // set by the writer thread
+    // fields an output needs; each output is a snapshot (device copy, then a
+    // pinned-memory copy that overlaps the following steps) that the writer
+    // thread reads and releases
+    unsigned fields = 0;
+    if (s.efdc == 1 || s.out_depth == 1) fields |= 1u << SW2D_FIELD_H;
+    if (s.efdc == 1) fields |= (1u << SW2D_FIELD_UUA) | (1u << SW2D_FIELD_VVA);
+    if (s.out_vx == 1) fields |= 1u << SW2D_FIELD_VV;
+    if (s.out_vy == 1) fields |= 1u << SW2D_FIELD_UU;
     double time = time0;
     long mstep = 0;
     int tq = -1;
@@ -267,59 +274,57 @@ int run_case(const Options &opt, const std::string &name)
         pending = 0;
         tq = tq + 1;
         tempo = (out0 > 0) ? tempo_name(round(mstep * dt / s.dkout)) : std::string("0");
-        if (!check(sw2d_get_field(h, SW2D_FIELD_H, hbuf.data()), "get_field")) break;
-        if (s.efdc == 1) {
-            const double efdc_time = mstep * dt / s.efdc_basetime;
-            const double *hh = (tq == 0) ? c.h0.data() : hbuf.data();
-            const double tt = (tq == 0) ? 0.0000 : efdc_time;
-            sw2d_couple_bc(R, C, res, res, c.mask.data(), hh, vva_host.data(), uua_host.data(), cap,
-                           cells.data(), I.data(), J.data(), q.data(), counts.data());
-            // records come back S, N, E, W; QSER.INP lists N, S, E, W
-            static const int kind_of_edge[4] = {1, 0, 2, 3};
-            for (int e = 0; e < 4; e++) {
-                const int k = kind_of_edge[e];
-                qser.nrec[e] = counts[k];
-                qser.time[e].push_back(counts[k] > 0 ? tt : 0.0);
-                qser.val[e].emplace_back(q.begin() + (size_t)k * cap, q.begin() + (size_t)k * cap + counts[k]);
+        long ticket = -1;
+        if (fields && !check(sw2d_snapshot(h, fields, 3600.0, &ticket), "snapshot")) break;
+        if (s.efdc == 1 && tq == 0 && !efdc_note) {
+            printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
+            efdc_note = true;
+        }
+        // the coupling state (records, QSER series, lagged velocities) and the
+        // files belong to the writer thread; jobs run in output order
+        writer.submit([&, ticket, tq, out0, efdc_time = mstep * dt / s.efdc_basetime,
+                       path = outdir + "Results_" + tempo + ".vtk"] {
+            auto field = [&](int f) -> const double * {
+                const double *p = nullptr;
+                if (sw2d_snapshot_field(h, ticket, f, &p) != SW2D_OK) write_failed = true;
+                return p;
+            };
+            const double *hnow = (fields >> SW2D_FIELD_H & 1u) ? field(SW2D_FIELD_H) : nullptr;
+            if (s.efdc == 1) {
+                const double *hh = (tq == 0) ? c.h0.data() : hnow;
+                const double tt = (tq == 0) ? 0.0000 : efdc_time;
+                const double *ua = field(SW2D_FIELD_UUA), *va = field(SW2D_FIELD_VVA);
+                if (hh && ua && va) {
+                    sw2d_couple_bc(R, C, res, res, c.mask.data(), hh, vva_host.data(), uua_host.data(), cap,
+                                   cells.data(), I.data(), J.data(), q.data(), counts.data());
+                    // records come back S, N, E, W; QSER.INP lists N, S, E, W
+                    static const int kind_of_edge[4] = {1, 0, 2, 3};
+                    for (int e = 0; e < 4; e++) {
+                        const int k = kind_of_edge[e];
+                        qser.nrec[e] = counts[k];
+                        qser.time[e].push_back(counts[k] > 0 ? tt : 0.0);
+                        qser.val[e].emplace_back(q.begin() + (size_t)k * cap,
+                                                 q.begin() + (size_t)k * cap + counts[k]);
+                    }
+                    // the velocities used by the next output are this
+                    // output's, taken after the coupling call (4913-4916)
+                    std::copy(ua, ua + RC, uua_host.begin());
+                    std::copy(va, va + RC, vva_host.begin());
+                    if (!sw2dio::write_qser(c.dir + "/EFDC/input/QSER.INP", 0, s.nlayers, qser,
+                                            s.efdc_basetime, tq))
+                        write_failed = true;
+                }
             }
-            // the velocities used by the next output are copied after the
-            // coupling call (4913-4916)
-            sw2d_get_field(h, SW2D_FIELD_UUA, uua_host.data());
-            sw2d_get_field(h, SW2D_FIELD_VVA, vva_host.data());
-            writer.submit([path = c.dir + "/EFDC/input/QSER.INP", nl = s.nlayers, snap = qser,
-                           base = s.efdc_basetime, tq, &write_failed] {
-                if (!sw2dio::write_qser(path, 0, nl, snap, base, tq)) write_failed = true;
-            });
-            if (tq == 0 && !efdc_note) {
-                printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
-                efdc_note = true;
-            }
-        }
-        // this output's fields, copied for the writer thread
-        const double *depth = (out0 == 0) ? c.h0.data() : hbuf.data();
-        std::vector<double> d_copy, vv_copy, uu_copy;
-        if (s.out_depth == 1) d_copy.assign(depth, depth + RC);
-        if (s.out_vx == 1) {
-            sw2d_get_field(h, SW2D_FIELD_VV, f1.data());
-            vv_copy = f1;
-        }
-        if (s.out_vy == 1) {
-            sw2d_get_field(h, SW2D_FIELD_UU, f2.data());
-            uu_copy = f2;
-        }
-        const bool probes = s.out_depth == 1 && s.out_outlet == 1;
-        const double *zz = (s.out_elev == 1) ? z.data() : nullptr;
-        writer.submit([&, d = std::move(d_copy), vv = std::move(vv_copy), uu = std::move(uu_copy),
-                       path = outdir + "Results_" + tempo + ".vtk", probes, zz] {
-            if (probes)
-                sw2dio::write_water_level(wl, R, C, c.dem.xll / res, c.dem.yll / res, outx, outy,
-                                          d.data());
+            const double *depth = (out0 == 0) ? c.h0.data() : hnow;
+            if (s.out_depth == 1 && s.out_outlet == 1 && depth)
+                sw2dio::write_water_level(wl, R, C, c.dem.xll / res, c.dem.yll / res, outx, outy, depth);
             sw2dio::VtkFields vf;
-            vf.depth = d.empty() ? nullptr : d.data();
-            vf.vv = vv.empty() ? nullptr : vv.data();
-            vf.uu = uu.empty() ? nullptr : uu.data();
-            vf.z = zz;
+            vf.depth = (s.out_depth == 1) ? depth : nullptr;
+            vf.vv = (s.out_vx == 1) ? field(SW2D_FIELD_VV) : nullptr;
+            vf.uu = (s.out_vy == 1) ? field(SW2D_FIELD_UU) : nullptr;
+            vf.z = (s.out_elev == 1) ? z.data() : nullptr;
             if (!sw2dio::write_vtk(path, C, R, res, res, c.dem.xll, c.dem.yll, vf)) write_failed = true;
+            if (ticket >= 0) sw2d_snapshot_release(h, ticket);
         });
     }
     if (pending > 0) check(sw2d_step(h, (int)pending), "step");

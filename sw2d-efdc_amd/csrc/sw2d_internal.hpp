@@ -161,6 +161,8 @@ inline int features_of(const StepCommon &s)
            (s.fixed_dt ? 0 : kFeatAdaptive) | (s.ncls_on ? kFeatMap : 0);
 }
 
+struct SnapRing;   // asynchronous output copies (sw2d_output.hip)
+
 }  // namespace sw2d
 
 struct sw2d_batch;
@@ -233,6 +235,7 @@ struct sw2d_handle {
     double kernel_ms = 0.0;
     long launches = 0;
     std::vector<void *> allocs;
+    sw2d::SnapRing *snap = nullptr;     // sw2d_snapshot slots, made on first use
     std::string err;
 };
 

@@ -96,6 +96,8 @@ int retile(sw2d_handle *h, int H);
 void advance(sw2d_handle *h, long m);
 int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
                 const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out);
+// defined in sw2d_output.hip
+void snapshot_free(sw2d_handle *h);
 // defined in sw2d_batch.hip
 int batch_fit_tiles(sw2d_batch *b, int *tile_rows, int *retiled);
 

@@ -121,6 +121,9 @@ def lib():
         "sw2d_sync": (ct.c_int, [_H]),
         "sw2d_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long), _DP]),
         "sw2d_get_field": (ct.c_int, [_H, ct.c_int, _DP]),
+        "sw2d_snapshot": (ct.c_int, [_H, ct.c_uint, _D, ct.POINTER(ct.c_long)]),
+        "sw2d_snapshot_field": (ct.c_int, [_H, ct.c_long, ct.c_int, ct.POINTER(_DP)]),
+        "sw2d_snapshot_release": (ct.c_int, [_H, ct.c_long]),
         "sw2d_couple_bc": (ct.c_int, [ct.c_int, ct.c_int, _D, _D, _IP, _DP, _DP, _DP, ct.c_int,
                                       _IP, _IP, _IP, _DP, _IP]),
         "sw2d_lake_catchment_smooth": (ct.c_int, [ct.c_int, ct.c_int, _IP, _DP]),
@@ -388,10 +391,41 @@ class Case:
     def time(self):
         return self.clock()[0]
 
-    def field(self, name):
-        out = np.zeros(self.R * self.C)
+    def field(self, name, out=None):
+        """sw2d_get_field; `out` (a C-contiguous float64 array of R*C values)
+        is filled in place when given."""
+        if out is None:
+            out = np.zeros(self.R * self.C)
+        if out.dtype != np.float64 or out.size != self.R * self.C or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array of rows*cols values")
         self._chk(self._L.sw2d_get_field(self._h, FIELDS[name], _dp(out)), "get_field")
         return out.reshape(self.R, self.C)
+
+    def snapshot(self, names, timeout=60.0):
+        """Queue an asynchronous copy of the named fields at the current step
+        (sw2d_snapshot); returns its ticket."""
+        mask = 0
+        for n in names:
+            mask |= 1 << FIELDS[n]
+        t = ct.c_long(-1)
+        self._chk(self._L.sw2d_snapshot(self._h, mask, timeout, ct.byref(t)), "snapshot")
+        return t.value
+
+    def snapshot_field(self, ticket, name, copy=True):
+        """Wait for a snapshot's copy; returns the field, copied out of the
+        pinned buffer, or with copy=False a read-only view of it that is valid
+        until the ticket is released."""
+        p = _DP()
+        self._chk(self._L.sw2d_snapshot_field(self._h, ticket, FIELDS[name], ct.byref(p)),
+                  "snapshot_field")
+        a = np.ctypeslib.as_array(p, shape=(self.R * self.C,)).reshape(self.R, self.C)
+        if copy:
+            return a.copy()
+        a.flags.writeable = False
+        return a
+
+    def snapshot_release(self, ticket):
+        self._chk(self._L.sw2d_snapshot_release(self._h, ticket), "snapshot_release")
 
     def balance(self):
         b = Balance()
