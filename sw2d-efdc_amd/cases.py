@@ -391,6 +391,27 @@ _EFDC = [
     (["# SALINITY #"], 0.0),
 ]
 
+# names of the EFDC block's values, in _EFDC order (the keys of write_db's
+# efdc_values)
+EFDC_NAMES = ["basetime", "TBEGIN", "NTSPTC", "nlayers", "ZBRADJ", "ZBRCVRT", "HMIN", "HADJ",
+              "HCVRT", "HDRY", "HWET", "BELADJ", "BELCVRT", "NWSER", "NASER", "IASWRAD", "REVC",
+              "RCHC", "SWRATNF", "SWRATNS", "FSWRATF", "DABEDT", "TBEDIT", "HTBED1", "HTBED2",
+              "ISPPH", "NPPPH", "ISRPPH", "IPPHXY", "ISVPH", "NPVPH", "ISRVPH", "IVPHXY", "NTSSSS",
+              "MTSP", "MTSC", "MTSA", "MTSUE", "MTSUT", "MTSU", "NSTYPE", "NSHOWR", "ISHOWC",
+              "JSHOWC", "NSHFREQ", "ZSSMIN", "ZSSMAX", "SSALMAX", "nprocessors", "lorp", "dye_on",
+              "temp_on", "salt_on", "DYE_INIT", "SALT_INIT", "TEMP_BOT", "TEMP_SURF", "DYE_IN",
+              "SALT_IN"]
+assert len(EFDC_NAMES) == len(_EFDC)
+
+
+def efdc_value(name, efdc_values=None, basetime=86400.0, nlayers=1):
+    """The value write_db puts in the EFDC block for `name`."""
+    i = EFDC_NAMES.index(name)
+    if efdc_values and name in efdc_values:
+        return efdc_values[name]
+    v = _EFDC[i][1]
+    return {"basetime": basetime, "nlayers": nlayers}.get(v, v) if isinstance(v, str) else v
+
 
 def _hms(t):
     d, r = divmod(int(round(t)), 86400)
@@ -411,10 +432,13 @@ def _esri(path, a, cellsize, nodata, fmt, xll=0.0, yll=0.0):
 
 def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=None,
              efdc=False, basetime=86400.0, nlayers=1, evaporation=None, INT=0.0, INF=0.0,
-             LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1)):
+             LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1), efdc_values=None, met=None):
     """Write db/dir.txt and db/<name>/ for a fixed-step spec in the reference's
     layout.  probes: (x, y) metres for coord_out.dat; sources: (xs, ys, Q, dtoq);
-    evaporation: (albedo, T[], Rg[]) with one value per rain interval."""
+    evaporation: (albedo, T[], Rg[]) with one value per rain interval.
+    efdc_values: EFDC block values by EFDC_NAMES name; met: (patm[], relh[],
+    wind[(speed, direction)]) for atm_press.dat, rel_humidity.dat and
+    wind_velocities.dat (constant series when None)."""
     import os
     base = os.path.join(root, "db", name)
     for d in ("SW2D/input", "SW2D/output", "EFDC/input", "EFDC/output"):
@@ -431,9 +455,9 @@ def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=N
                          efdc=1 if efdc else 0)
     if efdc:
         lines = []
-        for comments, v in _EFDC:
+        for (comments, v), nm in zip(_EFDC, EFDC_NAMES):
             lines += comments
-            val = {"basetime": basetime, "nlayers": nlayers}.get(v, v) if isinstance(v, str) else v
+            val = efdc_value(nm, efdc_values, basetime, nlayers)
             lines += ["", repr(float(val)) if isinstance(val, float) else str(val), ""]
         txt += "\n" + "\n".join(lines) + "\n"
     with open(os.path.join(base, "setup.dat"), "w") as f:
@@ -468,4 +492,12 @@ def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=N
         mask = spec.lake_mask if spec.lake_mask is not None else np.zeros(spec.z.shape, np.int32)
         _esri(os.path.join(inp, "mask.dat"), mask.astype(np.int64), spec.cellsize, -9999,
               lambda v: str(int(v)))
+        n = len(rain) + 2
+        patm, relh, wind = met if met else ([1000.0] * n, [0.8] * n, [(0.0, 0.0)] * n)
+        with open(os.path.join(inp, "atm_press.dat"), "w") as f:
+            f.write("atm_pressure(millibar)\n" + "".join(f"{float(v)!r}\n" for v in patm))
+        with open(os.path.join(inp, "rel_humidity.dat"), "w") as f:
+            f.write("rel_humidity(percent)\n" + "".join(f"{float(v)!r}\n" for v in relh))
+        with open(os.path.join(inp, "wind_velocities.dat"), "w") as f:
+            f.write("".join(f"{float(a)!r}\t{float(b)!r}\n" for a, b in wind))
     return base

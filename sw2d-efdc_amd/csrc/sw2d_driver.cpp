@@ -7,12 +7,15 @@
 // SW2D/output/Results_<k>.vtk, SW2D/output/WaterLevel.txt,
 // SW2D/output/TimeSimu_<k>.txt and, for coupled cases, the lake-inflow series
 // EFDC/input/QSER.INP (couple_BC_efdc + write_efdc_QSER, with the
-// reference's one-output lag of the velocities).  The EFDC side (EFDC.INP
-// card patching, the other EFDC input files, GEFDC/Gorp and the mpirun of the
-// Fortran lake model) is not part of this build.
+// reference's one-output lag of the velocities).  With the coupling on it
+// also writes EFDC's series and initial-condition inputs (WSER, ASER, SHOW,
+// DYE/SALT/TEMP.INP, DSER/TSER/SSER).  EFDC.INP card patching, GEFDC/Gorp and
+// the mpirun of the Fortran lake model are not part of this build.
 //
 // usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32]
-//                  [--device N] [--autotune] [--parse-only]
+//                  [--device N] [--autotune] [--parse-only] [--efdc-inputs]
+// --efdc-inputs writes the EFDC inputs known before the run (WSER, ASER,
+// SHOW, DYE/SALT/TEMP.INP) and stops; no GPU is used.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -92,6 +95,7 @@ struct Options {
     int device = 0;
     bool autotune = false;
     bool parse_only = false;
+    bool efdc_inputs = false;
 };
 
 bool parse_args(int argc, char **argv, Options &o)
@@ -110,6 +114,7 @@ bool parse_args(int argc, char **argv, Options &o)
         else if (a == "--device" && next(v)) o.device = atoi(v.c_str());
         else if (a == "--autotune") o.autotune = true;
         else if (a == "--parse-only") o.parse_only = true;
+        else if (a == "--efdc-inputs") o.efdc_inputs = true;
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return false;
@@ -147,6 +152,78 @@ void print_parse_summary(const sw2dio::Case &c, double time0, double timmax)
            [&] { long n = 0; for (int m : c.mask) n += (m == 1); return n; }());
 }
 
+// EFDC inputs the coupled run writes before stepping (4405-4425 wind,
+// 4588-4612 initial constituent fields, 4767-4785 atmospheric series).
+// Series shorter than the run's EFDC record count end the run with a
+// message (the reference reads past them).
+bool write_efdc_setup_files(const sw2dio::Case &c, const std::vector<double> &ev, double time0,
+                            double timmax, long lkout)
+{
+    const sw2dio::Setup &s = c.setup;
+    const std::vector<double> &blk = s.efdc_block;
+    const std::string in = c.dir + "/EFDC/input/";
+    const int R = c.dem.rows, C = c.dem.cols, nl = s.nlayers;
+    const int ntime = (int)(((timmax - time0) / s.dt) / (double)lkout);   // efdc_ntimes (4241)
+    auto wrote = [&](bool done, const char *file) {
+        if (!done) printf(" cannot write %s%s\n", in.c_str(), file);
+        return done;
+    };
+    auto short_series = [&](const char *what, size_t have, size_t need) {
+        printf(" %s holds %zu values, %zu needed for %d EFDC records\n", what, have, need, ntime);
+        return false;
+    };
+    if ((int)blk[sw2dio::kNWSER] == 1) {
+        if (c.wind.size() < 2 * (size_t)std::max(ntime, 0))
+            return short_series("wind_velocities.dat", c.wind.size() / 2, (size_t)ntime);
+        if (!wrote(sw2dio::write_wser(in + "WSER.INP", c.wind, c.dtrain, s.efdc_basetime, ntime), "WSER.INP"))
+            return false;
+    }
+    int water = 0;
+    for (int m : c.mask) water += (m == 1);
+    if ((int)blk[sw2dio::kDyeOn] == 1 &&
+        !wrote(sw2dio::write_initial_field(in + "DYE.INP", C, R, nl, water, c.mask,
+                                           [&](size_t, int) { return blk[sw2dio::kDyeInit]; }), "DYE.INP"))
+        return false;
+    if ((int)blk[sw2dio::kSaltOn] == 1 &&
+        !wrote(sw2dio::write_initial_field(in + "SALT.INP", C, R, nl, water, c.mask,
+                                           [&](size_t, int) { return blk[sw2dio::kSaltInit]; }), "SALT.INP"))
+        return false;
+    if ((int)blk[sw2dio::kTempOn] == 1) {
+        // temperature decreasing with depth below the surface value: layer l
+        // of the initial depth column sits (nl - 1 - l) / nl of it deep
+        const double tb = blk[sw2dio::kTempBot], ts = blk[sw2dio::kTempSurf];
+        auto temp = [&](size_t m, int l) {
+            const double L_DEP = (c.h0[m] / nl) * ((nl - 1) - l);
+            return tb + (ts - tb) / (1.00 + L_DEP);
+        };
+        if (!wrote(sw2dio::write_initial_field(in + "TEMP.INP", C, R, nl, water, c.mask, temp), "TEMP.INP"))
+            return false;
+    }
+    if (s.evaporation_on == 1) {
+        const size_t n = (size_t)std::max(ntime, 0);
+        if (c.patm.size() < n) return short_series("atm_press.dat", c.patm.size(), n);
+        if (c.relh.size() < n) return short_series("rel_humidity.dat", c.relh.size(), n);
+        if (c.T.size() < n || c.rain.size() < n || c.Rg.size() < n || ev.size() < n)
+            return short_series("temperatures/rain/solar_radiation.dat",
+                                std::min({c.T.size(), c.rain.size(), c.Rg.size(), ev.size()}), n);
+        std::vector<double> rows(7 * n);
+        for (size_t t = 0; t < n; t++) {
+            double *r = &rows[7 * t];
+            r[0] = c.patm[t];
+            r[1] = c.T[t];
+            r[2] = c.relh[t];
+            r[3] = (1 - s.LWL) * c.rain[t];
+            r[4] = ev[t];
+            r[5] = c.Rg[t];
+            r[6] = 0.00;   // cloud cover
+        }
+        if (!wrote(sw2dio::write_aser(in + "ASER.INP", rows, &blk[sw2dio::kIASWRAD], c.dtrain,
+                                      s.efdc_basetime, ntime), "ASER.INP"))
+            return false;
+    }
+    return true;
+}
+
 int run_case(const Options &opt, const std::string &name)
 {
     printf("%s\n", " ******************************************************************** ");
@@ -170,6 +247,23 @@ int run_case(const Options &opt, const std::string &name)
         timmax = ((double)c.rain.size() - 1) * c.dtrain;  // (cont_rain - 2) * dtrain
     if (opt.parse_only) {
         print_parse_summary(c, time0, timmax);
+        return 0;
+    }
+    if (opt.efdc_inputs) {
+        if (s.efdc != 1) {
+            printf(" the EFDC coupling is off in setup.dat: no EFDC inputs to write\n");
+            return 1;
+        }
+        std::vector<double> ev;
+        if (s.evaporation_on == 1) {
+            ev.resize(c.T.size());
+            sw2d_evaporation_series(c.albedo, c.T.data(), c.Rg.data(), (int)c.T.size(), c.dtrain, ev.data());
+        }
+        if (!write_efdc_setup_files(c, ev, time0, timmax, lkout)) return 1;
+        if (!sw2dio::write_show(c.dir + "/EFDC/input/SHOW.INP", &s.efdc_block[sw2dio::kNSTYPE])) {
+            printf(" cannot write %s/EFDC/input/SHOW.INP\n", c.dir.c_str());
+            return 1;
+        }
         return 0;
     }
 
@@ -203,11 +297,13 @@ int run_case(const Options &opt, const std::string &name)
     bool ok = check(sw2d_set_rain_series(h, c.rain.data(), (int)c.rain.size(), c.dtrain), "rain");
     ok = ok && check(sw2d_set_sources(h, c.src_x.data(), c.src_y.data(), (int)c.src_x.size(),
                                       c.Q.data(), c.nq, s.dtoq), "sources");
+    std::vector<double> ev;
     if (ok && s.evaporation_on == 1) {
-        std::vector<double> ev(c.T.size());
+        ev.resize(c.T.size());
         sw2d_evaporation_series(c.albedo, c.T.data(), c.Rg.data(), (int)c.T.size(), c.dtrain, ev.data());
         ok = check(sw2d_set_evaporation(h, ev.data(), (int)ev.size()), "evaporation");
     }
+    if (ok && s.efdc == 1) ok = write_efdc_setup_files(c, ev, time0, timmax, lkout);
     if (ok && opt.autotune) {
         int H = 0;
         ok = check(sw2d_autotune(h, 10, &H), "autotune");
@@ -277,7 +373,10 @@ int run_case(const Options &opt, const std::string &name)
         long ticket = -1;
         if (fields && !check(sw2d_snapshot(h, fields, 3600.0, &ticket), "snapshot")) break;
         if (s.efdc == 1 && tq == 0 && !efdc_note) {
-            printf(" EFDC.INP / SHOW.INP card updates are not written by this build\n");
+            // written with the first output, beside the EFDC.INP card update (4924-4938)
+            const double *sv = &s.efdc_block[sw2dio::kNSTYPE];
+            if (!sw2dio::write_show(c.dir + "/EFDC/input/SHOW.INP", sv)) write_failed = true;
+            printf(" EFDC.INP card updates are not written by this build\n");
             efdc_note = true;
         }
         // the coupling state (records, QSER series, lagged velocities) and the
@@ -330,8 +429,34 @@ int run_case(const Options &opt, const std::string &name)
     if (pending > 0) check(sw2d_step(h, (int)pending), "step");
     sw2d_sync(h);
     writer.finish();
+    // constituent series of the coupling records, after the run (5076-5092)
+    if (s.efdc == 1 && tq >= 0) {
+        const std::string in = c.dir + "/EFDC/input/";
+        const int nl = s.nlayers;
+        const double base = s.efdc_basetime;
+        const std::vector<double> &blk = s.efdc_block;
+        if ((int)blk[sw2dio::kDyeOn] == 1 &&
+            !sw2dio::write_constituent_series(in + "DSER.INP", 'D', 0, nl, qser,
+                                              [&](int) { return blk[sw2dio::kDyeIn]; }, base, tq))
+            write_failed = true;
+        if ((int)blk[sw2dio::kTempOn] == 1) {
+            // one temperature per output from temperatures.dat
+            if ((int)c.T.size() <= tq) {
+                printf(" TSER.INP needs %d values of temperatures.dat (evaporation on), %zu read\n",
+                       tq + 1, c.T.size());
+                write_failed = true;
+            } else if (!sw2dio::write_constituent_series(in + "TSER.INP", 'T', 0, nl, qser,
+                                                         [&](int t) { return c.T[t]; }, base, tq)) {
+                write_failed = true;
+            }
+        }
+        if ((int)blk[sw2dio::kSaltOn] == 1 &&
+            !sw2dio::write_constituent_series(in + "SSER.INP", 'S', 0, nl, qser,
+                                              [&](int) { return blk[sw2dio::kSaltIn]; }, base, tq))
+            write_failed = true;
+    }
     if (write_failed) {
-        printf(" an output file could not be written (VTK or QSER.INP)\n");
+        printf(" an output file could not be written (VTK, QSER.INP, SHOW.INP or a constituent series)\n");
         sw2d_destroy(h);
         return 1;
     }

@@ -202,6 +202,7 @@ static void read_setup(FILE *fp, Setup &s)
                 s.efdc_block.push_back(x);
             }
         }
+        if (s.efdc_block.size() < (size_t)kEfdcValues) s.efdc_block.resize(kEfdcValues, 0.0);
         s.efdc_basetime = s.efdc_block[0];
         s.TBEGIN = s.efdc_block[1];
         s.NTSPTC = (int)s.efdc_block[2];
@@ -356,6 +357,21 @@ std::string read_case(const std::string &root, const std::string &name, Case &c)
         c.mask.assign(n, 0);
         for (size_t i = 0; i < n; i++)
             if (fscanf(f.fp, "%d\n", &c.mask[i]) != 1) return "mask.dat holds fewer than nrows*ncols values";
+        // EFDC meteorological series (4427-4455): a header word, then one
+        // value per data interval; wind (4405-4425) as speed/direction pairs
+        auto series = [&](const char *name, const char *head, std::vector<double> &out, int per) {
+            File g(in + name, "r");
+            if (!g) return false;
+            if (head) scan_lit(g.fp, head);
+            double v[2];
+            while (fscanf(g.fp, per == 2 ? " %lf %lf\n " : " %lf\n ", &v[0], &v[1]) == per)
+                out.insert(out.end(), v, v + per);
+            return true;
+        };
+        if (!series("atm_press.dat", "atm_pressure(millibar)", c.patm, 1)) return "unknown file - atm_press.dat";
+        if (!series("rel_humidity.dat", "rel_humidity(percent)", c.relh, 1)) return "unknown file - rel_humidity.dat";
+        if ((int)c.setup.efdc_block[kNWSER] == 1 && !series("wind_velocities.dat", nullptr, c.wind, 2))
+            return "unknown file - wind_velocities.dat";
     }
     return "";
 }
@@ -458,6 +474,176 @@ bool write_qser(const std::string &path, int type, int nlayers, const QserSeries
             }
         }
     }
+    return true;
+}
+
+}  // namespace sw2dio
+
+namespace sw2dio {
+
+bool write_constituent_series(const std::string &path, char kind, int type, int nlayers,
+                              const QserSeries &q, const std::function<double(int)> &value,
+                              double efdc_basetime, int tq)
+{
+    File o(path, "w");
+    if (!o) return false;
+    FILE *fp = o.fp;
+    // the header of EFDC's constituent series files: the QSER layout with
+    // the constituent's letter in the series names
+    const char k = kind;
+    fprintf(fp, "C **, %cSER.INP Time Series FILE\nC **\nC **\n", k);
+    fprintf(fp, "C **  InType M%cSER(NS) TC%cSER(NS) TA%cSER(NS) RMULADJ(NS) ADDADJ(NS)\nC **\n", k, k, k);
+    fprintf(fp, "C **  IF InType.EQ.1 THEN READ DEPTH WEIGHTS AND SINGLE VALUE OF %cSER\n", k);
+    fprintf(fp, "C **                      ELSE READ A VALUE OF %cSER FOR EACH LAYER\nC **\n", k);
+    fprintf(fp, "C **  InType = 1 Structure\nC **  WKQ(K), K = 1, KC\n");
+    fprintf(fp, "C **  T%cSER(M, NS)    %cSER(M, 1, NS) !(M%cSER(NS) PAIRS FOR NS = 1, N%cSER SERIES)\nC **\n",
+            k, k, k, k);
+    fprintf(fp, "C **  InType = 0 Structure\n");
+    fprintf(fp, "C **  T%cSER(M, NS)    (%cSER(M, K, NS), K = 1, KC) !(M%cSER(NS) PAIRS)\nC **\n", k, k, k);
+    static const char *names[4] = {" ' *** North_edge_", " ' *** South_edge_", " ' *** East_edge_",
+                                   " ' *** West_edge_"};
+    int edg = 0;
+    for (int e = 0; e < 4; e++) {
+        for (int b = 0; b < q.nrec[e]; b++) {
+            edg++;
+            fprintf(fp, "      %d %d %lf %d %d %d %s%d\n", type, tq + 1, efdc_basetime, 0, 1, 0, names[e], edg);
+            for (int t = 0; t <= tq; t++) {
+                const double tt = t < (int)q.time[0].size() ? q.time[0][t] : 0.0;
+                fprintf(fp, "      %lf ", tt);
+                const double v = value(t);
+                for (int il = 0; il < nlayers; il++) fprintf(fp, il == nlayers - 1 ? "%lf\n" : "%lf ", v);
+            }
+        }
+    }
+    return true;
+}
+
+bool write_initial_field(const std::string &path, int cols, int rows, int nlayers, int water_cells,
+                         const std::vector<int> &mask,
+                         const std::function<double(size_t, int)> &value)
+{
+    File o(path, "w");
+    if (!o) return false;
+    FILE *fp = o.fp;
+    // the cell number's field width: the reference's expression of the digit
+    // count of the lake's cell count, minus one
+    const int width = water_cells > 0 ? (int)(round(1 + log((double)water_cells) / log(10.0)) - 1) : 0;
+    fprintf(fp, "C\nC\nC\nC\n1\n");
+    int cell = 0;
+    for (int jr = 0; jr < rows; jr++) {
+        const size_t row0 = (size_t)(rows - 1 - jr) * cols;   // south row first
+        for (int i = 0; i < cols; i++) {
+            const size_t m = row0 + i;
+            if (mask[m] != 1) continue;
+            cell++;
+            fprintf(fp, "%*d     %d     %d     ", width, cell, i + 1, jr + 1);
+            for (int l = 0; l < nlayers; l++) fprintf(fp, l < nlayers - 1 ? "%lf    " : "%lf\n", value(m, l));
+        }
+    }
+    return true;
+}
+
+bool write_wser(const std::string &path, const std::vector<double> &wind, double data_basetime,
+                double efdc_basetime, int ntime)
+{
+    if ((size_t)2 * std::max(ntime, 0) > wind.size()) return false;
+    File o(path, "w");
+    if (!o) return false;
+    FILE *fp = o.fp;
+    static const char *hdr[] = {
+        "C **, wser.inp Time Series FILE",
+        "C **",
+        "C **  WIND FORCING FILE, USE WITH 7 APRIL 97 AND LATER VERSIONS OF EFDC",
+        "C **",
+        "C **  MASER(NW) = NUMBER OF TIME DATA POINTS",
+        "C **  TCASER(NW) = DATA TIME UNIT CONVERSION TO SECONDS",
+        "C **  TAASER(NW) = ADDITIVE ADJUSTMENT OF TIME VALUES SAME UNITS AS INPUT TIMES",
+        "C **  WINDSCT(NW) = WIND SPEED CONVERSION TO M / SEC",
+        "C **  ISWDINT(NW) = DIRECTION CONVENTION",
+        "C **               0 DIRECTION TO",
+        "C **               1 DIRECTION FROM",
+        "C **               2 WINDS IS EAST VELOCITY, WINDD IS NORTH VELOCITY",
+        "#  EFDC_DS_WIND_HEIGHT: 10.0",
+        "#  EFDC_DS_ICE_PERIOD_START : 0.",
+        "#  EFDC_DS_ICE_PERIOD_END : 0.",
+        "C **  MASER  TCASER   TAASER  WINDSCT  ISWDINT",
+        "C **  TASER(M) WINDS(M) WINDD(M)",
+    };
+    for (const char *l : hdr) fprintf(fp, "%s\n", l);
+    fprintf(fp, "      %d %lf %lf %lf %d\n", ntime, efdc_basetime, 0.0, 1.0, 1);
+    for (int t = 0; t < ntime; t++)
+        fprintf(fp, "      %lf %lf %lf\n", ((double)t * data_basetime) / efdc_basetime, wind[2 * t], wind[2 * t + 1]);
+    return true;
+}
+
+bool write_aser(const std::string &path, const std::vector<double> &rows7, const double *thermal,
+                double data_basetime, double efdc_basetime, int ntime)
+{
+    if ((size_t)7 * std::max(ntime, 0) > rows7.size()) return false;
+    File o(path, "w");
+    if (!o) return false;
+    FILE *fp = o.fp;
+    static const char *hdr[] = {
+        "# ASER.INP ",
+        "# ",
+        "#\t",
+        "#  ATMOSPHERIC FORCING FILE, USE WITH 28 JULY 96 AND LATER VERSIONS OF EFDC",
+        "#  MASER = NUMBER OF TIME DATA POINTS",
+        "#  TCASER = DATA TIME UNIT CONVERSION TO SECONDS",
+        "#  TAASER = ADDITIVE ADJUSTMENT OF TIME VALUES SAME UNITS AS INPUT TIMES",
+        "#  IRELH = 0 VALUE TWET COLUMN VALUE IS TWET, = 1 VALUE IS RELATIVE HUMIDITY",
+        "#  RAINCVT = CONVERTS RAIN TO UNITS OF M / SEC.in / hr * 0.0254 m / 1 in * 1 hr / 3600 s = 7.05556E-06 (m / s)",
+        "#  EVAPCVT = CONVERTS EVAP TO UNITS OF M / SEC, IF EVAPCVT<0 EVAP IS INTERNALLY COMPUTED",
+        "#  SOLRCVT = CONVERTS SOLAR SW RADIATION TO JOULES / SQ METER",
+        "#  CLDCVT = MULTIPLIER FOR ADJUSTING CLOUD COVER",
+        "#  IASWRAD = 0 DISTRIBUTE SW SOL RAD OVER WATER COL AND INTO BED, = 1 ALL TO SURF LAYER",
+        "#  REVC = 1000 * EVAPORATIVE TRANSFER COEF, REVC<0 USE WIND SPD DEPD DRAG COEF",
+        "#  RCHC = 1000 * CONVECTIVE HEAT TRANSFER COEF, REVC<0 USE WIND SPD DEPD DRAG COEF",
+        "#  SWRATNF = FAST SCALE SOLAR SW RADIATION ATTENUATION COEFFCIENT 1. / METERS",
+        "#  SWRATNS = SLOW SCALE SOLAR SW RADIATION ATTENUATION COEFFCIENT 1. / METERS",
+        "#  FSWRATF = FRACTION OF SOLSR SW RADIATION ATTENUATED FAST  0<FSWRATF<1",
+        "#  DABEDT = DEPTH OR THICKNESS OF ACTIVE BED TEMPERATURE LAYER, METERS",
+        "#  TBEDIT = INITIAL BED TEMPERATURE",
+        "#  HTBED1 = CONVECTIVE HT COEFFCIENT BETWEEN BED AND BOTTOM WATER LAYER  NO DIM",
+        "#  HTBED2 = HEAT TRANS COEFFCIENT BETWEEN BED AND BOTTOM WATER LAYER  M / SEC",
+        "#  PATM = ATM PRESS MILLIBAR",
+        "#  TDRY / TEQ = DRY ATM TEMP ISOPT(2) = 1 OR EQUIL TEMP ISOPT(2) = 2",
+        "#  TWET / RELH = WET BULB ATM TEMP IRELH = 0, RELATIVE HUMIDITY IRELH = 1",
+        "#  RAIN = RAIN FALL RATE LENGTH / TIME",
+        "#  EVAP = EVAPORATION RATE IF EVAPCVT>0.",
+        "#  SOLSWR = SOLAR SHORT WAVE RADIATION AT WATER SURFACE  ENERGY FLUX / UNIT AREA",
+        "#  CLOUD = FRATIONAL CLOUD COVER",
+        "#",
+        "#  MASER    TCASER  TAASER  IRELH    RAINCVT  EVAPCVT  SOLRCVT   CLDCVT",
+        "#",
+        "#  IASWRAD  REVC    RCHC    SWRATNF  SWRATNS  FSWRATF  DABEDT    TBEDIT    HTBED1    HTBED2",
+        "#",
+        "#  TASER(M) PATM(M) TDRY(M) TWET(M)  RAIN(M)  EVAP(M)  SOLSWR(M) CLOUD(M)",
+        "#                    /TEQ   /RELH                      /HTCOEF",
+    };
+    for (const char *l : hdr) fprintf(fp, "%s\n", l);
+    const double cvt = 1 / (data_basetime * 1000.0);   // mm per interval -> m/s
+    fprintf(fp, "      %d %lf %lf %d %E %E %lf %lf\n", ntime, efdc_basetime, 1.0, 1, cvt, cvt, 1.0, 1.0);
+    const double *p = thermal;   // IASWRAD REVC RCHC SWRATNF SWRATNS FSWRATF DABEDT TBEDIT HTBED1 HTBED2
+    fprintf(fp, "      %d %lf %lf %lf %lf %E %lf %lf %E %E\n", (int)p[0], p[1], p[2], p[3], p[4], p[5],
+            p[6], p[7], p[8], p[9]);
+    for (int t = 0; t < ntime; t++) {
+        fprintf(fp, "      %lf ", ((double)t * data_basetime) / efdc_basetime);
+        for (int c = 0; c < 7; c++) fprintf(fp, c == 6 ? "%lf\n" : "%lf ", rows7[7 * (size_t)t + c]);
+    }
+    return true;
+}
+
+bool write_show(const std::string &path, const double *v)
+{
+    File o(path, "w");
+    if (!o) return false;
+    FILE *fp = o.fp;
+    fprintf(fp, "*  SHOW.INP FILE, IN FREE FORMAT ACROSS LINE\n*\n");
+    fprintf(fp, "*  NSTYPE    NSHOWR    ISHOWC   JSHOWC    NSHFREQ\n*\n");
+    fprintf(fp, "*  ZSSMIN    ZSSMAX    SSALMAX\n*\n");
+    fprintf(fp, "    %d    %d    %d    %d    %d\n", (int)v[0], (int)v[1], (int)v[2], (int)v[3], (int)v[4]);
+    fprintf(fp, "    %d    %d    %d\n", (int)v[5], (int)v[6], (int)v[7]);
     return true;
 }
 

@@ -6,6 +6,7 @@
 #pragma once
 #include <stdio.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,18 @@ struct Setup {
     double efdc_basetime = 0, TBEGIN = 0;
     int NTSPTC = 0, nlayers = 1;
     std::vector<double> efdc_block;  // every value of the block, in file order
+};
+
+// positions of named values in Setup::efdc_block: the order in which the
+// reference reads setup.dat's EFDC block (SW2D-EFDC.cu:3823-3972)
+enum EfdcValue {
+    kBaseTime = 0, kTBEGIN, kNTSPTC, kNLayers, kZBRADJ, kZBRCVRT, kHMIN, kHADJ, kHCVRT, kHDRY,
+    kHWET, kBELADJ, kBELCVRT, kNWSER, kNASER, kIASWRAD, kREVC, kRCHC, kSWRATNF, kSWRATNS,
+    kFSWRATF, kDABEDT, kTBEDIT, kHTBED1, kHTBED2, kISPPH, kNPPPH, kISRPPH, kIPPHXY, kISVPH,
+    kNPVPH, kISRVPH, kIVPHXY, kNTSSSS, kMTSP, kMTSC, kMTSA, kMTSUE, kMTSUT, kMTSU, kNSTYPE,
+    kNSHOWR, kISHOWC, kJSHOWC, kNSHFREQ, kZSSMIN, kZSSMAX, kSSALMAX, kNProcessors, kLorpType,
+    kDyeOn, kTempOn, kSaltOn, kDyeInit, kSaltInit, kTempBot, kTempSurf, kDyeIn, kSaltIn,
+    kEfdcValues
 };
 
 struct Grid {
@@ -47,6 +60,8 @@ struct Case {
     std::vector<double> out_x, out_y;  // coord_out.dat (metres)
     std::vector<int> mask;             // mask.dat (efdc == 1)
     Grid mask_hdr;                     // mask.dat header (efdc == 1)
+    std::vector<double> patm, relh;    // atm_press.dat, rel_humidity.dat (efdc == 1)
+    std::vector<double> wind;          // wind_velocities.dat pairs (efdc == 1, NWSER == 1)
 };
 
 // count of '\n' in a file (the reference sizes series this way, 4041-4048)
@@ -81,5 +96,34 @@ struct QserSeries {
 };
 bool write_qser(const std::string &path, int type, int nlayers, const QserSeries &q,
                 double efdc_basetime, int tq);
+
+// DSER.INP / TSER.INP / SSER.INP ('D', 'T', 'S'; write_efdc_DSER 2383-2498,
+// TSER 2141-2254, SSER 2026-2140): one series per coupling record of q, in
+// QSER's record order, value(t) in every layer at outputs 0..tq.  Every
+// series takes its times from the north edge's list (the reference prints
+// north_val[t][0] for all four edges).
+bool write_constituent_series(const std::string &path, char kind, int type, int nlayers,
+                              const QserSeries &q, const std::function<double(int)> &value,
+                              double efdc_basetime, int tq);
+
+// DYE.INP / SALT.INP / TEMP.INP (2255-2382): one line per lake cell
+// (mask == 1), south row first, numbered in that order; value(cell, layer)
+// with cell the memory index (north row first).
+bool write_initial_field(const std::string &path, int cols, int rows, int nlayers, int water_cells,
+                         const std::vector<int> &mask,
+                         const std::function<double(size_t, int)> &value);
+
+// WSER.INP (write_efdc_WSER 2612-2662): ntime (speed, direction) rows of wind
+bool write_wser(const std::string &path, const std::vector<double> &wind, double data_basetime,
+                double efdc_basetime, int ntime);
+
+// ASER.INP (write_efdc_ASER 2664-2743): rows of 7 values (PATM, TDRY, RELH,
+// RAIN, EVAP, SOLSWR, CLOUD); the ten thermal parameters in block order
+// from kIASWRAD
+bool write_aser(const std::string &path, const std::vector<double> &rows7, const double *thermal,
+                double data_basetime, double efdc_basetime, int ntime);
+
+// SHOW.INP (write_efdc_SHOW 2745-2764): the eight values from kNSTYPE
+bool write_show(const std::string &path, const double *show);
 
 }  // namespace sw2dio

@@ -1,9 +1,11 @@
 """The sw2d-efdc drop-in driver: the reference's db/ file contract.
 
-CPU tests parse generated cases (--parse-only).  The GPU test runs a whole
-coupled case and compares every output file with the same files built from
-the CPU FP64 oracle, formatted with the reference's printf formats
-(SW2D-EFDC.cu:4958-5072 for VTK and WaterLevel.txt, 2499-2610 for QSER.INP).
+CPU tests parse generated cases (--parse-only) and check the EFDC inputs
+known before a run (--efdc-inputs).  The GPU test runs a whole coupled case
+and compares every output file with the same files built from the CPU FP64
+oracle, formatted with the reference's printf formats (SW2D-EFDC.cu:4958-5072
+for VTK and WaterLevel.txt, 2499-2610 for QSER.INP, 2026-2498 for the
+constituent series).
 """
 import json
 import os
@@ -32,7 +34,11 @@ CASE_KW = dict(dkout=120.0, dpout=60.0, probes=[(55.0, 105.0), (125.0, 35.0), (3
                sources=([105.0], [155.0], [[0.5], [1.0], [0.2], [0.0]], 150.0), efdc=True,
                nlayers=2, basetime=3600.0,
                evaporation=(0.2, [20.0, 21.0, 22.0, 23.0, 24.0], [300.0, 400, 500, 600, 700]),
-               INT=0.1, INF=0.05, LWL=0.2, EV_WL_min=0.01)
+               INT=0.1, INF=0.05, LWL=0.2, EV_WL_min=0.01,
+               efdc_values=dict(dye_on=1, temp_on=1, salt_on=1, DYE_IN=10.0, SALT_IN=15.0,
+                                DYE_INIT=0.5, SALT_INIT=25.0, TEMP_BOT=5.0, TEMP_SURF=23.0,
+                                NWSER=1, NSTYPE=3, NSHOWR=7200, ISHOWC=4, JSHOWC=5, NSHFREQ=720,
+                                ZSSMIN=-1, ZSSMAX=1300, SSALMAX=1000))
 
 
 def run_driver(root, *args, check=True):
@@ -68,6 +74,112 @@ def test_setup_crlf_and_whitespace(cases, tmp_path):
     open(p, "w", newline="").write(txt)
     j = json.loads(run_driver(tmp_path, "--parse-only").stdout.strip().splitlines()[-1])
     assert (j["dt"], j["manning"], j["efdc_basetime"], j["nlayers"]) == (0.5, 0.03, 3600.0, 2)
+
+
+# ---- EFDC series and initial-condition inputs (formats of 2026-2498, 2612-2765) ----
+
+def c_round_half_away(v):
+    return float(np.sign(v) * np.floor(abs(v) + 0.5))
+
+
+def initial_field_text(mask, nlayers, value):
+    """DYE/SALT/TEMP.INP: lake cells south row first, numbered in that order;
+    the number's width is round(1 + ln(n)/ln(10)) - 1 for n lake cells."""
+    R, C = mask.shape
+    n = int((mask == 1).sum())
+    width = int(c_round_half_away(1 + np.log(n) / np.log(10)) - 1)
+    out = ["C", "C", "C", "C", "1"]
+    cell = 0
+    for jr in range(R):
+        r = R - 1 - jr
+        for i in range(C):
+            if mask[r, i] != 1:
+                continue
+            cell += 1
+            vals = [value(r, i, l) for l in range(nlayers)]
+            out.append("%*d     %d     %d     " % (width, cell, i + 1, jr + 1)
+                       + "    ".join("%f" % v for v in vals))
+    return "\n".join(out) + "\n"
+
+
+def series_text(kind, times, nrec, value, nlayers, basetime):
+    k = kind
+    hdr = [f"C **, {k}SER.INP Time Series FILE", "C **", "C **",
+           f"C **  InType M{k}SER(NS) TC{k}SER(NS) TA{k}SER(NS) RMULADJ(NS) ADDADJ(NS)", "C **",
+           f"C **  IF InType.EQ.1 THEN READ DEPTH WEIGHTS AND SINGLE VALUE OF {k}SER",
+           f"C **                      ELSE READ A VALUE OF {k}SER FOR EACH LAYER", "C **",
+           "C **  InType = 1 Structure", "C **  WKQ(K), K = 1, KC",
+           f"C **  T{k}SER(M, NS)    {k}SER(M, 1, NS) !(M{k}SER(NS) PAIRS FOR NS = 1, N{k}SER SERIES)",
+           "C **", "C **  InType = 0 Structure",
+           f"C **  T{k}SER(M, NS)    ({k}SER(M, K, NS), K = 1, KC) !(M{k}SER(NS) PAIRS)", "C **"]
+    lines = list(hdr)
+    edg = 0
+    ntq = len(times)
+    for label, n in zip(("North", "South", "East", "West"), nrec):
+        for _ in range(n):
+            edg += 1
+            lines.append("      0 %d %f 0 1 0  ' *** %s_edge_%d" % (ntq, basetime, label, edg))
+            for t in range(ntq):
+                lines.append("      %f " % times[t] + " ".join(["%f" % value(t)] * nlayers))
+    return "\n".join(lines) + "\n"
+
+
+def test_efdc_inputs_written_before_the_run(cases, po, tmp_path):
+    """--efdc-inputs writes WSER, ASER, SHOW and the DYE/SALT/TEMP initial
+    fields from the case files alone (no GPU): values and layout against a
+    restatement of the reference's writers (wind pairs and meteorological
+    series per data interval, efdc_ntimes = run length / dt / lkout rows)."""
+    spec = small_case(cases)
+    rng = np.random.default_rng(5)
+    n = len(spec.rain) + 2
+    met = (1000.0 + rng.uniform(-20, 20, n), rng.uniform(0.3, 0.9, n),
+           [(a, b) for a, b in zip(rng.uniform(0, 8, n), rng.uniform(0, 360, n))])
+    kw = dict(CASE_KW, nlayers=3)
+    base = pathlib.Path(cases.write_db(spec, tmp_path, "C6", met=met, **kw))
+    run_driver(tmp_path, "--efdc-inputs")
+    inp = base / "EFDC" / "input"
+    ev = kw["efdc_values"]
+    nl, basetime, dtrain = 3, kw["basetime"], spec.dtrain
+    ntime = int((spec.t_end / spec.dt) / round(kw["dkout"] / spec.dt))
+    mask = spec.lake_mask
+    assert (inp / "DYE.INP").read_text() == initial_field_text(mask, nl, lambda r, i, l: ev["DYE_INIT"])
+    assert (inp / "SALT.INP").read_text() == initial_field_text(mask, nl, lambda r, i, l: ev["SALT_INIT"])
+    tb, ts = ev["TEMP_BOT"], ev["TEMP_SURF"]
+    temp = lambda r, i, l: tb + (ts - tb) / (1.0 + (spec.h0[r, i] / nl) * ((nl - 1) - l))
+    assert (inp / "TEMP.INP").read_text() == initial_field_text(mask, nl, temp)
+    # WSER: 17 comment lines, the series header, then (time in base units, speed, direction)
+    w = (inp / "WSER.INP").read_text().splitlines()
+    assert len(w) == 17 + 1 + ntime and all(l.startswith(("C **", "#")) for l in w[:17])
+    assert w[17] == "      %d %f %f %f %d" % (ntime, basetime, 0.0, 1.0, 1)
+    for t in range(ntime):
+        assert w[18 + t] == "      %f %f %f" % (t * dtrain / basetime, *met[2][t])
+    # ASER: 36 comment lines, two parameter lines, then 7 columns per interval
+    a = (inp / "ASER.INP").read_text().splitlines()
+    assert len(a) == 36 + 2 + ntime and all(l.startswith("#") for l in a[:36])
+    cvt = 1 / (dtrain * 1000.0)
+    assert a[36] == "      %d %f %f %d %E %E %f %f" % (ntime, basetime, 1.0, 1, cvt, cvt, 1.0, 1.0)
+    th = [cases.efdc_value(k, ev) for k in ("IASWRAD", "REVC", "RCHC", "SWRATNF", "SWRATNS",
+                                            "FSWRATF", "DABEDT", "TBEDIT", "HTBED1", "HTBED2")]
+    assert a[37] == "      %d %f %f %f %f %E %f %f %E %E" % (int(th[0]), *th[1:])
+    albedo, T, Rg = kw["evaporation"]
+    evap = po.evaporation(albedo, T, Rg, dtrain)
+    for t in range(ntime):
+        row = [met[0][t], T[t], met[1][t], (1 - kw["LWL"]) * spec.rain[t], evap[t], Rg[t], 0.0]
+        assert a[38 + t] == "      %f " % (t * dtrain / basetime) + " ".join("%f" % v for v in row), t
+    sh = (inp / "SHOW.INP").read_text().splitlines()
+    assert sh[6:] == ["    3    7200    4    5    720", "    -1    1300    1000"]
+    assert len(sh) == 8 and all(l.startswith("*") for l in sh[:6])
+
+
+def test_efdc_inputs_short_series_are_reported(cases, tmp_path):
+    """A wind file shorter than the run's EFDC record count ends the run with
+    a message (the reference reads past the end of its array)."""
+    spec = small_case(cases)
+    n = len(spec.rain) + 2
+    met = ([1000.0] * n, [0.5] * n, [(1.0, 90.0)] * 2)
+    cases.write_db(spec, tmp_path, "C7", met=met, **CASE_KW)
+    r = run_driver(tmp_path, "--efdc-inputs", check=False)
+    assert r.returncode == 1 and "wind_velocities.dat holds 2 values" in r.stdout
 
 
 def test_missing_input_is_reported(cases, tmp_path):
@@ -177,6 +289,16 @@ def test_driver_outputs_match_oracle(cases, po, tmp_path):
     got = (pathlib.Path(base) / "EFDC" / "input" / "QSER.INP").read_text()
     assert got == qser_text(hist, CASE_KW["nlayers"], CASE_KW["basetime"])
     assert any(v > 0 for r in hist[1:] for e in ("north", "south", "east", "west") for v in r[e]["q"])
+    # constituent series after the run: QSER's records, the north edge's times
+    nrec = [len(hist[-1][e]["q"]) for e in ("north", "south", "east", "west")]
+    times = [r["time"] if nrec[0] > 0 else 0.0 for r in hist]
+    ev, nl, bt = CASE_KW["efdc_values"], CASE_KW["nlayers"], CASE_KW["basetime"]
+    inp = pathlib.Path(base) / "EFDC" / "input"
+    T = CASE_KW["evaporation"][1]
+    assert (inp / "DSER.INP").read_text() == series_text("D", times, nrec, lambda t: ev["DYE_IN"], nl, bt)
+    assert (inp / "SSER.INP").read_text() == series_text("S", times, nrec, lambda t: ev["SALT_IN"], nl, bt)
+    assert (inp / "TSER.INP").read_text() == series_text("T", times, nrec, lambda t: T[t], nl, bt)
+    assert (inp / "SHOW.INP").is_file() and (inp / "TEMP.INP").is_file()
 
 
 @pytest.mark.gpu
