@@ -227,7 +227,9 @@ int sw2d_get_field(sw2d_handle *h, int field, double *out);
  * SW2D_E_STATE.  sw2d_snapshot_field waits for the copy and gives rows*cols
  * doubles valid until sw2d_snapshot_release.  sw2d_snapshot_field and
  * sw2d_snapshot_release may be called from another thread (a writer) while
- * this one steps the handle. */
+ * this one steps the handle, but not during sw2d_destroy; they return a code
+ * without setting the handle's error text (which belongs to the stepping
+ * thread). */
 int sw2d_snapshot(sw2d_handle *h, unsigned fields, double timeout_s, long *ticket);
 int sw2d_snapshot_field(sw2d_handle *h, long ticket, int field, const double **data);
 int sw2d_snapshot_release(sw2d_handle *h, long ticket);
