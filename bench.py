@@ -562,8 +562,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is described in DESIGN.md, not built)"}))
+        print(json.dumps({"error": "this library runs on one GPU; --gpus must be 1 "
+                                   "(sharding one grid or a batch over GPUs is not built)"}))
         sys.exit(2)
 
     pkg = load_package()
