@@ -56,3 +56,26 @@ def test_evaporation_series_matches_oracle(pkg, po):
     a = pkg.sw2d.evaporation_series(0.12, T, Rg, 3600.0)
     b = po.evaporation(0.12, T, Rg, 3600.0)
     assert np.array_equal(a, b)
+
+
+def test_snapshot_argument_errors(pkg):
+    """The asynchronous output calls reject a NULL handle before any device work."""
+    L = pkg.sw2d.lib()
+    t = ct.c_long(-1)
+    assert L.sw2d_snapshot(None, 1, 0.0, ct.byref(t)) == -1
+    p = ct.POINTER(ct.c_double)()
+    assert L.sw2d_snapshot_field(None, 0, 0, ct.byref(p)) == -1
+    assert L.sw2d_snapshot_release(None, 0) == -1
+
+
+def test_bench_runs_on_one_gpu_only():
+    """bench.py --gpus N with N > 1 exits with a message instead of measuring."""
+    import json
+    import pathlib
+    import subprocess
+    import sys
+    root = pathlib.Path(__file__).resolve().parents[1]
+    r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
