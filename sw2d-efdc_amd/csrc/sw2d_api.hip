@@ -312,6 +312,15 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     if (params->adaptive && !(params->courant > 0)) return fail(nullptr, SW2D_E_ARG, "courant must be positive");
     if (params->tile_rows < 0 || params->tile_rows > 512)
         return fail(nullptr, SW2D_E_ARG, "tile_rows must be in [0, 512]");
+    {
+        // cell offsets in the kernels are 32-bit: the padded grid (pitch a
+        // multiple of 16) and the plain layout (R*C + C/2 plus 2C + 4 of tail)
+        // must stay below 2^31 elements
+        const long long R = grid->rows, C = grid->cols;
+        const long long padded = R * ((C + 15) / 16 * 16), flat = R * C + C / 2 + 2 * C + 4;
+        if (padded >= (1LL << 31) || flat >= (1LL << 31))
+            return fail(nullptr, SW2D_E_ARG, "grid too large: rows * cols must stay below 2^31 cells");
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
         return fail(nullptr, SW2D_E_NODEV, "no HIP device");

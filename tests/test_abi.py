@@ -79,3 +79,19 @@ def test_bench_runs_on_one_gpu_only():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_create_rejects_grids_beyond_32bit_offsets(pkg):
+    """Cell offsets in the kernels are 32-bit: a grid of 2^31 or more cells
+    (padded) is refused before any array is read or any device work."""
+    sw = pkg.sw2d
+    L = sw.lib()
+    z = np.zeros(4)
+    p = sw.Params()
+    L.sw2d_default_params(ct.byref(p))
+    out = ct.c_void_p()
+    g = sw.GridDesc(46341, 46341, 0.0, 0.0, 10.0, -9999.0)   # 2.1e9 cells
+    assert L.sw2d_create(ct.byref(g), sw._dp(z), sw._dp(z), ct.byref(p), ct.byref(out)) == -1
+    assert b"too large" in L.sw2d_last_error(None)
+    g = sw.GridDesc(65536, 32760, 0.0, 0.0, 10.0, -9999.0)   # pitch 32768: 2^31 padded cells
+    assert L.sw2d_create(ct.byref(g), sw._dp(z), sw._dp(z), ct.byref(p), ct.byref(out)) == -1
