@@ -58,7 +58,9 @@ def test_device_math_bitwise(pkg, po):
     a = 10 ** rng.uniform(-300, 300, 20000)
     b = 10 ** rng.uniform(-300, 300, 20000) * rng.choice([-1, 1], 20000)
     assert np.array_equal(sw.device_math(3, a), np.sqrt(a))
-    assert np.array_equal(sw.device_math(4, a, b), a / b)
+    with np.errstate(over="ignore", under="ignore"):   # quotients past the range are part of the check
+        q = a / b
+    assert np.array_equal(sw.device_math(4, a, b), q)
     nanx = np.array([np.nan, 1.0, -2.0, np.nan])
     assert np.array_equal(sw.device_math(5, nanx, np.zeros(4)), np.array([0.0, 1.0, 0.0, 0.0]))
 
