@@ -468,7 +468,7 @@ int run_case(const Options &opt, const std::string &name)
         fclose(ts);
     }
     fclose(wl);
-    if (s.efdc == 1) printf(" the EFDC lake model is not run by this build (QSER.INP is ready)\n");
+    if (s.efdc == 1) printf(" the EFDC lake model is not run by this build (its inputs are in EFDC/input)\n");
     sw2d_destroy(h);
     return 0;
 }
