@@ -314,11 +314,12 @@ def serve_leg(pkg, cases, seconds, rate, hours, plain_rates, active_cells=0, til
     }
 
 
-def config0_leg(pkg, cases, threads):
+def config0_leg(pkg, cases):
     """The reference's own CPU-runnable case (256x256, dx 10 m, 1 h of rain and
-    recession, fixed dt 0.5 s, 3600 iterations), the whole run: the FP64 fused
-    kernel (graph replays) against the FP64 oracle on the host cores, and the
-    final states compared value for value."""
+    recession, fixed dt 0.5 s, 3600 iterations), the whole run on the GPU:
+    the FP64 fused kernel (graph replays).  Returns the leg's figures and
+    (spec, iterations, final depth) for the CPU baseline leg, which times the
+    FP64 oracle on the same run and compares the final states."""
     spec = cases.config0(outlet=True)
     R, C = spec.z.shape
     iters = int(round(spec.t_end / (2 * spec.dt)))
@@ -338,28 +339,13 @@ def config0_leg(pkg, cases, threads):
     gpu_s = time.perf_counter() - t0
     h_gpu = c.field("h")
     c.close()
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import pyoracle as po
-    o = po.Oracle(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning,
-                  threads=threads)
-    o.set_rain(spec.rain, spec.dtrain)
-    o.set_outlet(*spec.outlet)
-    t0 = time.perf_counter()
-    o.step(iters)
-    cpu_s = time.perf_counter() - t0
-    same = bool(np.array_equal(h_gpu, o.field("h")))
     return {
         "workload": "config 0: 256x256, dx 10 m, 1 % slope + fBm 2 m, n 0.03, rain 50 mm/h for "
                     "30 min then 30 min recession, fixed dt 0.5 s, outlet weir; 3600 iterations",
         "iterations": iters,
         "gpu_fp64_s": gpu_s,
         "gpu_cell_updates_per_s": R * C * iters / gpu_s,
-        "cpu_oracle_s": cpu_s,
-        "cpu_threads": threads,
-        "cpu_cell_updates_per_s": R * C * iters / cpu_s,
-        "speedup": cpu_s / gpu_s,
-        "final_h_equal": same,
-    }
+    }, (spec, iters, h_gpu)
 
 
 def output_leg(pkg, spec, iters, every, precision="fp64"):
@@ -463,11 +449,27 @@ def pond_break_leg(pkg, cases, side, sample=30.0):
     }
 
 
-def cpu_baseline(spec, seconds, threads):
+def cpu_baseline(spec, seconds, threads, config0=None):
     """Time the FP64 oracle (OpenMP over rows) on the same case for about
-    `seconds` of CPU work."""
+    `seconds` of CPU work.  With config0 = (spec, iterations, final GPU depth)
+    it also times the oracle's whole run of that case and checks the GPU's
+    final depths against it value for value."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as po
+    c0 = None
+    if config0 is not None:
+        spec0, iters0, h_gpu = config0
+        o0 = po.Oracle(spec0.z, spec0.h0, spec0.cellsize, dt=spec0.dt, manning=spec0.manning,
+                       threads=threads)
+        o0.set_rain(spec0.rain, spec0.dtrain)
+        o0.set_outlet(*spec0.outlet)
+        t0 = time.perf_counter()
+        o0.step(iters0)
+        cpu_s = time.perf_counter() - t0
+        c0 = {"cpu_oracle_s": cpu_s, "cpu_threads": threads,
+              "cpu_cell_updates_per_s": spec0.cells * iters0 / cpu_s,
+              "final_h_equal": bool(np.array_equal(h_gpu, o0.field("h")))}
+        del o0
     o = po.Oracle(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning,
                   threads=threads)
     if spec.rain is not None:
@@ -497,10 +499,11 @@ def cpu_baseline(spec, seconds, threads):
         el = time.perf_counter() - t0
         if el >= seconds:
             break
-    return dict(value=spec.cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
-                sample=f"{n} iterations of the same {spec.z.shape[0]}x{spec.z.shape[1]} case "
-                       f"({el:.1f} s), FP64 oracle, OpenMP {threads} threads",
-                cpu_model=cpu_model(), host_cpus=os.cpu_count())
+    out = dict(value=spec.cells * n / el, unit="cell-updates/s", cores=threads, kind="port",
+               sample=f"{n} iterations of the same {spec.z.shape[0]}x{spec.z.shape[1]} case "
+                      f"({el:.1f} s), FP64 oracle, OpenMP {threads} threads",
+               cpu_model=cpu_model(), host_cpus=os.cpu_count())
+    return out, c0
 
 
 def cpu_model():
@@ -698,16 +701,19 @@ def main():
             f"{out['serving_sustained']['p50_time_to_hydrograph_s']:.2f} s, p99 "
             f"{out['serving_sustained']['p99_time_to_hydrograph_s']:.2f} s")
 
+    c0_run = None
     if not args.no_cpu and args.config0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        out["config0"] = config0_leg(pkg, cases, threads)
-        log(f"config 0: GPU {out['config0']['gpu_fp64_s']:.3f} s, CPU oracle "
-            f"{out['config0']['cpu_oracle_s']:.2f} s ({threads} threads), final h equal: "
-            f"{out['config0']['final_h_equal']}")
+        out["config0"], c0_run = config0_leg(pkg, cases)
+        log(f"config 0: GPU {out['config0']['gpu_fp64_s']:.3f} s")
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads, {args.cpu_seconds} s")
-        out["cpu_baseline"] = cpu_baseline(spec, args.cpu_seconds, threads)
+        out["cpu_baseline"], c0_cpu = cpu_baseline(spec, args.cpu_seconds, threads, c0_run)
+        if c0_cpu is not None:
+            out["config0"].update(c0_cpu)
+            out["config0"]["speedup"] = c0_cpu["cpu_oracle_s"] / out["config0"]["gpu_fp64_s"]
+            log(f"config 0: CPU oracle {c0_cpu['cpu_oracle_s']:.2f} s ({threads} threads), "
+                f"final h equal: {c0_cpu['final_h_equal']}")
     else:
         out["cpu_baseline"] = None
     if args.calibrate_copy_mb > 0:
