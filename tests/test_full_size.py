@@ -70,3 +70,23 @@ def test_pond_break_4096_tiling_invariance_and_ledger(pkg, cases):
     assert b["volume"] == pytest.approx(h.sum() * spec.cellsize ** 2, rel=1e-12)
     assert b["volume0"] == pytest.approx(spec.h0.sum() * spec.cellsize ** 2, rel=1e-12)
     assert (h[spec.h0 == 0] > 0).any()           # the front has left the reservoir
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lake_at_rest_2048_exact(pkg, precision):
+    """A flat free surface (eta = 10 m) over an uneven bed on a 1/8 m grid,
+    no forcing, closed walls: every face flux stays exactly 0 and every depth
+    exactly its initial value, at full size and in both forms (depths,
+    surfaces and bed steps are exact binary fractions, so both the FP64
+    surface test and the FP32 bed-step form see a zero gradient)."""
+    R = C = 2048
+    rng = np.random.default_rng(42)
+    z = np.floor(rng.uniform(0.0, 8.0, (R, C)) * 8.0) / 8.0
+    h0 = 10.0 - z
+    c = pkg.Case(z, h0, 5.0, dt=0.5, manning=0.03, precision=precision)
+    c.step(100)
+    assert np.array_equal(c.field("h"), h0)
+    assert not c.field("um").any() and not c.field("vn").any()
+    b = c.balance()
+    assert b["error"] == 0.0 or abs(b["error"]) <= 1e-15 * b["volume"]
+    c.close()
