@@ -90,3 +90,33 @@ def test_lake_at_rest_2048_exact(pkg, precision):
     b = c.balance()
     assert b["error"] == 0.0 or abs(b["error"]) <= 1e-15 * b["volume"]
     c.close()
+
+
+def test_batch_of_large_uneven_grids_equals_stand_alone(pkg, cases):
+    """Four large scenarios of uneven shapes (about 3.4 M cells together) in
+    one batch, FP32 form, 60 iterations: each member's depths and fluxes equal
+    its stand-alone run value for value (one launch's tile table then spans
+    tens of thousands of blocks across the members)."""
+    shapes = [(1024, 1024), (768, 1280), (512, 512), (2048, 256)]
+    batch = pkg.Batch(precision="fp32", pool_bytes=2 << 30)
+    members, alone = [], []
+    for i, (R, C) in enumerate(shapes):
+        r = np.arange(R, dtype=np.float64)[:, None]
+        z = np.broadcast_to(0.01 * 10.0 * (R - 1 - r) + cases.fbm(R, C, 0.7, 1.5, 300 + i), (R, C)).copy()
+        h0 = np.zeros_like(z)
+        rain = np.array([40.0 + 10 * i, 20.0])
+        a = pkg.Case(z, h0, 10.0, dt=0.5, manning=0.04, precision="fp32")
+        a.set_rain(rain, 60.0)
+        a.step(60)
+        alone.append([a.field(n) for n in ("h", "um", "vn")])
+        a.close()
+        m = batch.case(z, h0, 10.0, dt=0.5, manning=0.04)
+        m.set_rain(rain, 60.0)
+        batch.attach(m, max_iters=60)
+        members.append(m)
+    while batch.step(6):
+        pass
+    for i, m in enumerate(members):
+        for name, want in zip(("h", "um", "vn"), alone[i]):
+            assert np.array_equal(m.field(name), want), (shapes[i], name)
+    batch.close()
