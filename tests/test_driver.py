@@ -242,10 +242,13 @@ def qser_text(hist, nlayers, basetime):
 
 
 @pytest.mark.gpu
-def test_driver_outputs_match_oracle(cases, po, tmp_path):
+@pytest.mark.parametrize("kernel", ["fused", "plain"])
+def test_driver_outputs_match_oracle(cases, po, tmp_path, kernel):
+    """Every output file of a coupled run, with the fused kernel and with the
+    reference's unfused sequence, equals the files built from the oracle."""
     spec = small_case(cases)
     base = cases.write_db(spec, tmp_path, "C4", **CASE_KW)
-    run_driver(tmp_path)
+    run_driver(tmp_path, "--kernel", kernel)
     R, C, dx = spec.z.shape[0], spec.z.shape[1], spec.cellsize
     z = po.lake_catchment_smooth(spec.lake_mask, spec.z)
     o = po.Oracle(z, spec.h0, dx, dt=spec.dt, manning=spec.manning, INT=0.1, INF=0.05, LWL=0.2,
