@@ -318,3 +318,46 @@ def test_driver_reports_an_output_write_failure(cases, tmp_path):
     assert r.returncode == 1
     assert "could not be written" in r.stdout
     assert (outdir / "Results_1.vtk").is_file()   # the other outputs were written
+
+
+def vtk_block(text, name):
+    """Values of one SCALARS block of a STRUCTURED_POINTS file (file order)."""
+    lines = text.splitlines()
+    i = lines.index(f"SCALARS {name} float 1") + 2
+    n = int(next(l for l in lines if l.startswith("POINT_DATA")).split()[1])
+    return np.array([float(v) for v in lines[i:i + n]])
+
+
+@pytest.mark.gpu
+def test_driver_fp32_depths_within_a_millimetre(cases, po, tmp_path):
+    """--precision fp32 (FP32 state, FP64 balance) through the whole drop-in
+    run: every VTK depth within 1 mm of the oracle's FP64 depth at the same
+    output (the files print 6 decimals)."""
+    spec = small_case(cases)
+    base = cases.write_db(spec, tmp_path, "C8", **CASE_KW)
+    run_driver(tmp_path, "--precision", "fp32")
+    R, C, dx = spec.z.shape[0], spec.z.shape[1], spec.cellsize
+    z = po.lake_catchment_smooth(spec.lake_mask, spec.z)
+    o = po.Oracle(z, spec.h0, dx, dt=spec.dt, manning=spec.manning, INT=0.1, INF=0.05, LWL=0.2,
+                  EV_WL_min=0.01)
+    o.set_rain(spec.rain, spec.dtrain)
+    xs, ys, Q, dtoq = CASE_KW["sources"]
+    o.set_sources([po.source_index(R, C, 0.0, 0.0, dx, xs[0], ys[0])], Q, dtoq)
+    albedo, T, Rg = CASE_KW["evaporation"]
+    o.set_evaporation(po.evaporation(albedo, T, Rg, spec.dtrain))
+    dt, lkout = spec.dt, round(CASE_KW["dkout"] / spec.dt)
+    time, mstep, out0, worst, n = 0.0, 0, -1, 0.0, 0
+    outdir = pathlib.Path(base) / "SW2D" / "output"
+    while time + dt <= spec.t_end:
+        out0 += 1
+        o.step(1)
+        time += 2 * dt
+        mstep += 2
+        if not (mstep % lkout == 0 or out0 == 0) or out0 == 0:
+            continue
+        tempo = str(int(round(mstep * dt / CASE_KW["dkout"])))
+        got = vtk_block((outdir / f"Results_{tempo}.vtk").read_text(), "depth")
+        want = o.field("h")[::-1].ravel()
+        worst = max(worst, float(np.max(np.abs(got - want))))
+        n += 1
+    assert n == 4 and worst <= 1e-3, (n, worst)
