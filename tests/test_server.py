@@ -226,3 +226,28 @@ def test_adaptive_tiling_keeps_results(pkg, cases):
         for k in ("volume", "v_source", "v_clip", "v_outflow"):
             assert r["balance"][k] == pytest.approx(bal[k], rel=1e-12, abs=1e-9), (i, k)
     srv.close()
+
+
+def test_fp64_server_with_adaptive_tiling_equals_alone(pkg, cases):
+    """The FP64 (parity) form served with adaptive tiling: each hydrograph
+    equals the stand-alone FP64 run's value for value."""
+    reqs = [request(cases, i) for i in (0, 1, 2, 5)]
+    srv = pkg.Server(precision="fp64", pool_bytes=256 << 20, tile_rows=0, slice_iters=12,
+                     max_active_cells=2 * max(z.size for z, *_ in reqs))
+    ids = [srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                      hydro_cap=CAP, **kw) for z, outlet, rain, kw, t_end in reqs]
+    srv.wait(timeout=300)
+    for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
+        r = srv.poll(ids[i])
+        assert r["state"] == "done"
+        c = pkg.Case(z, np.zeros_like(z), 10.0, precision="fp64", **kw)
+        c.set_rain(rain, 900.0)
+        c.set_outlet(*outlet)
+        c.set_hydrograph(CAP)
+        c.run_until(t_end)
+        assert np.array_equal(srv.hydrograph(ids[i]), c.hydrograph()), f"request {i}"
+        bal = c.balance()
+        for k in ("volume", "v_source", "v_clip", "v_outflow"):
+            assert r["balance"][k] == pytest.approx(bal[k], rel=1e-12, abs=1e-9), (i, k)
+        c.close()
+    srv.close()
