@@ -359,8 +359,8 @@ std::string read_case(const std::string &root, const std::string &name, Case &c)
             if (fscanf(f.fp, "%d\n", &c.mask[i]) != 1) return "mask.dat holds fewer than nrows*ncols values";
         // EFDC meteorological series (4427-4455): a header word, then one
         // value per data interval; wind (4405-4425) as speed/direction pairs
-        auto series = [&](const char *name, const char *head, std::vector<double> &out, int per) {
-            File g(in + name, "r");
+        auto series = [&](const char *file, const char *head, std::vector<double> &out, int per) {
+            File g(in + file, "r");
             if (!g) return false;
             if (head) scan_lit(g.fp, head);
             double v[2];
