@@ -361,3 +361,27 @@ def test_driver_fp32_depths_within_a_millimetre(cases, po, tmp_path):
         worst = max(worst, float(np.max(np.abs(got - want))))
         n += 1
     assert n == 4 and worst <= 1e-3, (n, worst)
+
+
+@pytest.mark.parametrize("ncells", [1, 10, 100])
+def test_efdc_initial_fields_edge_widths(cases, tmp_path, ncells):
+    """DYE/SALT/TEMP.INP numbering widths at the powers of ten (1 lake cell
+    gives width 0, 10 and 100 cells the next widths) with one layer."""
+    spec = small_case(cases)
+    R, C = spec.z.shape
+    mask = np.zeros((R, C), np.int32)
+    idx = np.random.default_rng(ncells).choice(R * C, ncells, replace=False)
+    mask.ravel()[idx] = 1
+    spec.lake_mask = mask
+    kw = dict(CASE_KW, nlayers=1)
+    kw["efdc_values"] = dict(kw["efdc_values"], NWSER=0)
+    base = pathlib.Path(cases.write_db(spec, tmp_path, "C9", **kw))
+    run_driver(tmp_path, "--efdc-inputs")
+    ev = kw["efdc_values"]
+    got = (base / "EFDC" / "input" / "DYE.INP").read_text()
+    assert got == initial_field_text(mask, 1, lambda r, i, l: ev["DYE_INIT"])
+    assert len(got.splitlines()) == 5 + ncells
+    tb, ts = ev["TEMP_BOT"], ev["TEMP_SURF"]
+    temp = lambda r, i, l: tb + (ts - tb) / (1.0 + (spec.h0[r, i] / 1) * (0 - l))
+    assert (base / "EFDC" / "input" / "TEMP.INP").read_text() == initial_field_text(mask, 1, temp)
+    assert not (base / "EFDC" / "input" / "WSER.INP").exists()   # NWSER = 0
