@@ -390,3 +390,165 @@ def test_derived_fields_of_linear_state(po):
     hd[4, 5] = 0.5e-4
     o.set_state(hd, um, vn)
     assert o.field("uu")[4, 5] == 0.0 and o.field("vv")[4, 5] == 0.0
+
+
+# ---- B4 momentum update (SW2D-EFDC.cu:165-202 for um, 282-319 for vn) ----
+# Each case sets a state whose one-iteration update has a closed form: the
+# gravity term u13, the upwind convection terms u11/u12 (v11/v12) and the
+# Manning friction ram, one at a time.  Dyadic inputs keep the convection
+# cases exact, so a wrong neighbour, a central instead of an upwind
+# difference or a flipped sign changes the result.
+
+def momentum_step(po, h, um, vn, *, dx, dt=0.5, n=0.0):
+    """One iteration on a flat bed from (h, um, vn); the new face fluxes."""
+    R, C = h.shape
+    o = po.Oracle(np.zeros((R, C)), h, dx, dt=dt, manning=n)
+    o.set_state(h, um, vn)
+    o.step(1)
+    return o.field("um"), o.field("vn")
+
+
+def grid_rc(R, C):
+    r = np.arange(R, dtype=np.float64)[:, None] + np.zeros((1, C))
+    j = np.arange(C, dtype=np.float64)[None, :] + np.zeros((R, 1))
+    return r, j
+
+
+@pytest.mark.parametrize("H,n,u,v", [(2.0, 0.05, 0.3, 0.0), (2.0, 0.05, 0.3, -0.2),
+                                     (0.05, 0.08, 0.02, 0.015), (0.4, 0.03, -0.1, 0.25)])
+def test_b4_friction_decay(po, H, n, u, v):
+    """Uniform depth H on a flat bed with uniform face fluxes (u, v): the
+    convection and gravity terms vanish and each face decays by the implicit
+    Manning factor, um' = (1 - dt2 ram / 2) um / (1 + dt2 ram / 2) with
+    ram = g n^2 |velocity| / H^1.3333330, |velocity| = hypot(u, v) / H (the
+    cross velocity enters through vva for um and uua for vn)."""
+    R, C = 8, 9
+    h = np.full((R, C), H)
+    um = np.full((R, C), u)
+    um[0] = 0.0                         # the north edge face carries no flux
+    vn = np.full((R, C), v)
+    vn[:, 0] = 0.0                      # nor does the west edge face
+    um1, vn1 = momentum_step(po, h, um, vn, dx=10.0, dt=0.5, n=n)
+    dt2 = 1.0
+    ram = G * n * n * (math.hypot(u, v) / H) / math.pow(H, 1.3333330)
+    k = dt2 * ram * 0.5
+    inner = np.s_[2:R - 1, 2:C - 1]     # faces whose stencils see only the uniform fluxes
+    np.testing.assert_allclose(um1[inner], (1 - k) * u / (1 + k), rtol=1e-13, atol=0)
+    np.testing.assert_allclose(vn1[inner], (1 - k) * v / (1 + k), rtol=1e-13, atol=0)
+    # the decay is real: friction takes at least 0.01 % off each flux
+    assert np.all(np.abs(um1[inner]) < np.abs(u) * (1 - 1e-4))
+
+
+@pytest.mark.parametrize("s", [1.0, -1.0])
+def test_b4_upwind_convection_along_flow(po, s):
+    """n = 0, flat surface, um linear in the row, um = s (a + b r) with the
+    flow all one way: the upwind flux difference is
+    u11 = (dt2/dx) (2 beta m - s beta^2 / 2), beta = s b, m = um at the face.
+    A central difference drops the beta^2 term, a downwind one flips it.
+    The vn mirror v12 (vn linear in the column) has the same closed form."""
+    R, C, dx = 10, 7, 8.0
+    lam = 1.0 / dx                     # dt2 / dx with dt2 = 1
+    r, j = grid_rc(R, C)
+    a, b = 0.25, 0.125
+    m = s * (a + b * r)
+    um = m.copy()
+    um[0] = 0.0
+    um1, vn1 = momentum_step(po, np.ones((R, C)), um, np.zeros((R, C)), dx=dx)
+    beta = s * b
+    expect = m - lam * (2 * beta * m - s * beta * beta / 2)
+    assert np.array_equal(um1[2:R - 1], expect[2:R - 1])
+    assert np.all(vn1 == 0)
+    # vn: linear in the column, the west edge face at 0
+    w = s * (a + b * j)
+    vn = w.copy()
+    vn[:, 0] = 0.0
+    um1, vn1 = momentum_step(po, np.ones((R, C)), np.zeros((R, C)), vn, dx=dx)
+    expect = w - lam * (2 * beta * w - s * beta * beta / 2)
+    assert np.array_equal(vn1[:, 2:C - 1], expect[:, 2:C - 1])
+    assert np.all(um1 == 0)
+
+
+@pytest.mark.parametrize("V", [0.5, -0.5])
+def test_b4_upwind_cross_convection(po, V):
+    """n = 0, flat surface.  um quadratic in the column, uniform down the rows,
+    advected by a uniform cross velocity vn = V: u11 = 0 and
+    u12 = (dt2/dy) V (um[j] - um[j-1]) for V > 0, (dt2/dy) V (um[j+1] - um[j])
+    for V < 0.  The vn mirror: vn quadratic in the row advected by a uniform
+    um = V gives v11 the same way (rows instead of columns)."""
+    R, C, dx = 9, 10, 8.0
+    lam = 1.0 / dx
+    r, j = grid_rc(R, C)
+    q = lambda x: 0.25 + 0.125 * x + x * x / 64.0    # noqa: E731  dyadic, positive
+    um = q(j)
+    um[0] = 0.0
+    vn = np.full((R, C), V)
+    vn[:, 0] = 0.0
+    um1, vn1 = momentum_step(po, np.ones((R, C)), um, vn, dx=dx)
+    jj = j[2:R - 1, 1:C - 1]
+    diff = q(jj) - q(jj - 1) if V > 0 else q(jj + 1) - q(jj)
+    assert np.array_equal(um1[2:R - 1, 1:C - 1], q(jj) - lam * V * diff)
+    # the advecting flux itself sees no gradient: unchanged
+    assert np.array_equal(vn1[2:R - 1, 2:C - 1], vn[2:R - 1, 2:C - 1])
+    # vn mirror: v11 from a uniform row-direction velocity
+    vn = q(r)
+    vn[:, 0] = 0.0
+    um = np.full((R, C), V)
+    um[0] = 0.0
+    um1, vn1 = momentum_step(po, np.ones((R, C)), um, vn, dx=dx)
+    rr = r[1:R - 1, 2:C - 1]
+    diff = q(rr) - q(rr - 1) if V > 0 else q(rr + 1) - q(rr)
+    assert np.array_equal(vn1[1:R - 1, 2:C - 1], q(rr) - lam * V * diff)
+    assert np.array_equal(um1[2:R - 1, 2:C - 1], um[2:R - 1, 2:C - 1])
+
+
+def test_b4_gravity_term(po):
+    """Fluid at rest under a sloping free surface over a flat bed: the first
+    update is the pressure gradient alone, um' = -g hm (dt2/dx) dh with
+    hm = the face depth (mean of the two cells) and dh = h(south) - h(north);
+    likewise vn' = -g hn (dt2/dy) (h(east) - h(west)).  Flow runs down the
+    surface: northwards (um < 0) when the surface rises southwards."""
+    R, C, dx = 8, 9, 10.0
+    r, j = grid_rc(R, C)
+    s_r, s_c = 0.0625, -0.03125
+    h = 1.0 + s_r * r + s_c * j
+    um1, vn1 = momentum_step(po, h, np.zeros((R, C)), np.zeros((R, C)), dx=dx, n=0.03)
+    lam = 1.0 / dx
+    np.testing.assert_allclose(um1[1:], (-G * (h[1:] - 0.5 * s_r) * lam * s_r), rtol=1e-14)
+    np.testing.assert_allclose(vn1[:, 1:], (-G * (h[:, 1:] - 0.5 * s_c) * lam * s_c), rtol=1e-14)
+    assert np.all(um1[1:] < 0) and np.all(vn1[:, 1:] > 0)
+    assert np.all(um1[0] == 0) and np.all(vn1[:, 0] == 0)
+
+
+def test_manning_normal_depth_on_plane(po):
+    """Steady rain i on a one-cell-wide plane of slope S draining to a free
+    outlet (the second column is NODATA).  At steady state each row face
+    carries the rain of the cells above it, um = i dx r, and gravity balances
+    the reference's friction g n^2 |u| u / hm^(4/3) (ram), which is Manning's
+    law um = hm^(5/3) sqrt(S_f) / n with the friction slope S_f.  Against the
+    local surface slope S_f = -(eta_r - eta_(r-1)) / dx that balance holds to
+    the convection term (< 1 %); against the bed slope S it is Manning's
+    normal depth, within the few per cent the surface curvature adds.  A
+    friction law with another depth exponent or n instead of n^2 misses by
+    tens of per cent.  The plane is gentle enough (S dx = 1 cm, depths
+    1.3-3.4 cm) that every face takes the momentum branch, not the dry-bed
+    weirs (a surface below the next bed)."""
+    R, dx, S, n, i_mm_h = 150, 2.0, 0.005, 0.03, 100.0
+    r = np.arange(R, dtype=np.float64)
+    z = np.stack([S * dx * (R - 1 - r), np.full(R, -9999.0)], axis=1)
+    o = po.Oracle(z, np.zeros((R, 2)), dx, dt=0.5, manning=n)
+    # one interval longer than the run: the series never reaches an interval
+    # boundary (where stream_flow's interpolation divides 0 by 0)
+    o.set_rain(np.full(3, i_mm_h * 100.0), 360000.0)
+    o.set_outlet(R - 1, 0)
+    o.step(10800)                         # 3 h; the wave crosses the plane in ~15 min
+    i = i_mm_h / 1000.0 / 3600.0
+    um, hm, h = o.field("um")[:, 0], o.field("hm")[:, 0], o.field("h")[:, 0]
+    rows = np.arange(R // 5, R - 5)
+    assert np.all(h[rows] > S * dx)       # no face is a weir
+    np.testing.assert_allclose(um[rows], i * dx * rows, rtol=1e-9)
+    eta = h + z[:, 0]
+    s_f = -(eta[rows] - eta[rows - 1]) / dx
+    np.testing.assert_allclose(um[rows], hm[rows] ** (5.0 / 3.0) * np.sqrt(s_f) / n, rtol=0.01)
+    np.testing.assert_allclose(um[rows], hm[rows] ** (5.0 / 3.0) * math.sqrt(S) / n, rtol=0.03)
+    led = o.ledger()
+    assert led["q_out"] == pytest.approx(i * dx * dx * R, rel=1e-9)
