@@ -219,6 +219,20 @@ int sw2d_time(sw2d_handle *h, double *time, long *mstep, double *dt2_next);
  * 5042, 5058).  Synchronises. */
 int sw2d_get_field(sw2d_handle *h, int field, double *out);
 
+/* sw2d_get_field without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same values as sw2d_get_field, but a derived field of a
+ * fused handle is staged through a fixed 16 MiB device array that each GPU
+ * holds from the library's load until the process ends, in chunks of whole
+ * rows, instead of the R*C buffer sw2d_get_field allocates on first use.
+ * Grids with more than 2^21 columns are refused (SW2D_E_ARG).  Calls on one
+ * GPU share its array: the library serialises them within a process (a call
+ * holds the array until its copy has finished), so calls that overlap on one
+ * GPU wait for each other.  Synchronises; restores the caller's current
+ * device. */
+int sw2d_get_field_noalloc(sw2d_handle *h, int field, double *out);
+
 /* Asynchronous output copies (double-buffered; stand-alone handles).  Same
  * values as sw2d_get_field, but the blocking copies at output times (4880-4916,
  * 4976-5072) become a snapshot that overlaps the steps that follow:

@@ -955,8 +955,8 @@ int sw2d_get_field(sw2d_handle *h, int field, double *out)
         // derived fields go through an R*C FP64 buffer made on first use
         int rc;
         if (!h->scratch && (rc = dalloc(h, &h->scratch, RC))) return rc;
-        hipError_t e = h->esize == 8 ? launch_derive_f64(h, field, h->scratch)
-                                     : launch_derive_f32(h, field, h->scratch);
+        hipError_t e = h->esize == 8 ? launch_derive_f64(h, field, h->scratch, 0, (long)RC)
+                                     : launch_derive_f32(h, field, h->scratch, 0, (long)RC);
         if (e != hipSuccess) return hip_fail(h, e, "derive");
         HIPCHK(h, hipMemcpyAsync(out, h->scratch, RC * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));

@@ -17,7 +17,10 @@ hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const Tile
 }
 int fused_waves_per_cu_f64(int features) { return fused_waves_per_cu_impl<double>(features); }
 hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double>(h); }
-hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out) { return launch_derive_impl<double>(h, field, out); }
+hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out, long first, long count)
+{
+    return launch_derive_impl<double>(h, field, out, first, count);
+}
 
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc)
 {

@@ -581,15 +581,18 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
         atomicMin(&clk[0].cfl_bits, (unsigned long long)__double_as_longlong(cfl));
 }
 
-// one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva)
+// one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva) over
+// the cells first .. first + count - 1 of the row-major rows x cols order;
+// out[i] receives cell first + i
 template <typename T>
 __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
                                                     const double *z, int R, int C, int P,
-                                                    double th, int field, double *out)
+                                                    double th, int field, double *out,
+                                                    long first, long count)
 {
     PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
-    const long n = (long)R * C;
-    for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+    const long n = first + count;
+    for (long idx = first + (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long)gridDim.x * blockDim.x) {
         const int r = (int)(idx / C), j = (int)(idx % C);
         const long o = (long)r * P + j;
@@ -609,7 +612,7 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
         case SW2D_FIELD_Z: v = z[o]; break;
         default: break;
         }
-        out[idx] = v;
+        out[idx - first] = v;
     }
 }
 
@@ -669,14 +672,16 @@ static hipError_t launch_init_impl(sw2d_handle *h)
 }
 
 template <typename T>
-static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev)
+static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev, long first,
+                                     long count)
 {
-    const long n = (long)h->R * h->C;
-    int blocks = (int)((n + 255) / 256);
+    int blocks = (int)((count + 255) / 256);
     if (blocks > 8192) blocks = 8192;
+    if (blocks < 1) return hipSuccess;
     hipLaunchKernelGGL(derive_field<T>, dim3(blocks), dim3(256), 0, h->stream,
                        (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
-                       (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev);
+                       (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev, first,
+                       count);
     return hipGetLastError();
 }
 

@@ -282,8 +282,10 @@ hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const Tile
                            int phase, int features);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
-hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev);
-hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev);
+// one derived field of cells first .. first + count - 1 (row-major, rows x
+// cols) into out_dev[0 .. count - 1]
+hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev, long first, long count);
+hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev, long first, long count);
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc);
 int fused_waves_per_cu_f64(int features);  // resident one-wave blocks per CU
 int fused_waves_per_cu_f32(int features);

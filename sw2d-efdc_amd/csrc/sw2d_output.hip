@@ -11,6 +11,7 @@
 // give double buffering: a writer formats one snapshot while the next is in
 // flight; a third snapshot waits until a slot is released.
 #include <condition_variable>
+#include <memory>
 #include <mutex>
 
 #include "sw2d_ops.hpp"
@@ -145,7 +146,8 @@ int sw2d_snapshot(sw2d_handle *h, unsigned fields, double timeout_s, long *ticke
     for (int f = 0; f < kSnapFields && e == hipSuccess; f++) {
         if (!(fields >> f & 1u)) continue;
         if (fused(h))
-            e = h->esize == 8 ? launch_derive_f64(h, f, s->dev[f]) : launch_derive_f32(h, f, s->dev[f]);
+            e = h->esize == 8 ? launch_derive_f64(h, f, s->dev[f], 0, (long)RC)
+                              : launch_derive_f32(h, f, s->dev[f], 0, (long)RC);
         else
             e = hipMemcpyAsync(s->dev[f], plain_field(h, f), RC * 8, hipMemcpyDeviceToDevice, h->stream);
     }
@@ -209,6 +211,104 @@ int sw2d_snapshot_release(sw2d_handle *h, long ticket)
     s->ticket = -1;
     r->freed.notify_all();
     return e == hipSuccess ? SW2D_OK : SW2D_E_HIP;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// sw2d_get_field_noalloc: sw2d_get_field staged through a fixed device array
+// instead of a buffer the library allocates.  The array is a file-scope
+// device variable: every device the library runs on holds its own copy, at
+// its own address, from module load until the process ends.  A field is
+// derived into it and copied out in row chunks, so any grid whose row fits
+// is served.  Host code never passes the variable's name to a kernel or a
+// copy (in host code it names a host-side stand-in): it takes the device
+// address from hipGetSymbolAddress, once per device, with that device
+// current.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+constexpr size_t kNoAllocDoubles = (size_t)2 << 20;   // 16 MiB of FP64 values per device
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+struct NoAllocDevice {
+    double *ptr = nullptr;   // device address of `scratch` on this device
+    std::mutex use;          // one call at a time on one device
+};
+
+std::mutex g_noalloc_mu;
+std::vector<std::unique_ptr<NoAllocDevice>> g_noalloc;
+
+// the device's staging area; looked up on the first call on that device
+// (the caller has made `device` current)
+int noalloc_area(sw2d_handle *h, int device, NoAllocDevice **out)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc.size() <= device) g_noalloc.resize(device + 1);
+    if (!g_noalloc[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return hip_fail(h, e, "hipGetSymbolAddress(scratch)");
+        g_noalloc[device] = std::make_unique<NoAllocDevice>();
+        g_noalloc[device]->ptr = (double *)p;
+    }
+    *out = g_noalloc[device].get();
+    return SW2D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sw2d_get_field_noalloc(sw2d_handle *h, int field, double *out)
+{
+    if (!h) return SW2D_E_ARG;
+    if (!out) return fail(h, SW2D_E_ARG, "out is NULL");
+    if (field < 0 || field > SW2D_FIELD_VV1) return fail(h, SW2D_E_ARG, "unknown field");
+    if (fused(h) && (size_t)h->C > kNoAllocDoubles)
+        return fail(h, SW2D_E_ARG, "grid rows wider than the 2 Mi-value staging area of "
+                                   "sw2d_get_field_noalloc; use sw2d_get_field");
+    int prev = 0;
+    HIPCHK(h, hipGetDevice(&prev));
+    HIPCHK(h, hipSetDevice(h->device));
+    int rc = SW2D_OK;
+    const size_t RC = (size_t)h->R * h->C;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+        rc = hip_fail(h, e, "hipStreamSynchronize");
+    } else if (!fused(h)) {
+        // the plain layout holds every field in a device array already
+        e = hipMemcpy(out, plain_field(h, field), RC * 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = hip_fail(h, e, "hipMemcpy");
+    } else {
+        NoAllocDevice *area = nullptr;
+        rc = noalloc_area(h, h->device, &area);
+        if (rc == SW2D_OK) {
+            std::lock_guard<std::mutex> lk(area->use);
+            // whole rows per chunk; chunk k + 1 is derived after the copy of
+            // chunk k in stream order, so one staging area serves them all
+            const long rows_per = (long)(kNoAllocDoubles / (size_t)h->C);
+            for (long r0 = 0; r0 < h->R && e == hipSuccess; r0 += rows_per) {
+                const long nr = std::min<long>(rows_per, h->R - r0);
+                const long first = r0 * h->C, count = nr * h->C;
+                e = h->esize == 8 ? launch_derive_f64(h, field, area->ptr, first, count)
+                                  : launch_derive_f32(h, field, area->ptr, first, count);
+                if (e == hipSuccess)
+                    e = hipMemcpyAsync(out + first, area->ptr, (size_t)count * 8,
+                                       hipMemcpyDeviceToHost, h->stream);
+            }
+            const hipError_t e2 = hipStreamSynchronize(h->stream);
+            if (e == hipSuccess) e = e2;
+            if (e != hipSuccess) rc = hip_fail(h, e, "get_field_noalloc");
+        }
+        if (rc == SW2D_OK && field == SW2D_FIELD_Z)
+            for (size_t i = 0; i < RC; i++)
+                if (out[i] != out[i]) out[i] = h->grid.nodata;
+    }
+    (void)hipSetDevice(prev);
+    return rc;
 }
 
 }  // extern "C"

@@ -121,6 +121,7 @@ def lib():
         "sw2d_sync": (ct.c_int, [_H]),
         "sw2d_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long), _DP]),
         "sw2d_get_field": (ct.c_int, [_H, ct.c_int, _DP]),
+        "sw2d_get_field_noalloc": (ct.c_int, [_H, ct.c_int, _DP]),
         "sw2d_snapshot": (ct.c_int, [_H, ct.c_uint, _D, ct.POINTER(ct.c_long)]),
         "sw2d_snapshot_field": (ct.c_int, [_H, ct.c_long, ct.c_int, ct.POINTER(_DP)]),
         "sw2d_snapshot_release": (ct.c_int, [_H, ct.c_long]),
@@ -399,6 +400,17 @@ class Case:
         if out.dtype != np.float64 or out.size != self.R * self.C or not out.flags.c_contiguous:
             raise ValueError("out must be a C-contiguous float64 array of rows*cols values")
         self._chk(self._L.sw2d_get_field(self._h, FIELDS[name], _dp(out)), "get_field")
+        return out.reshape(self.R, self.C)
+
+    def field_noalloc(self, name, out=None):
+        """sw2d_get_field_noalloc: the same values as field(), staged through
+        the library's fixed per-device array (no device allocation)."""
+        if out is None:
+            out = np.zeros(self.R * self.C)
+        if out.dtype != np.float64 or out.size != self.R * self.C or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous float64 array of rows*cols values")
+        self._chk(self._L.sw2d_get_field_noalloc(self._h, FIELDS[name], _dp(out)),
+                  "get_field_noalloc")
         return out.reshape(self.R, self.C)
 
     def snapshot(self, names, timeout=60.0):
