@@ -96,6 +96,10 @@ struct Options {
     bool autotune = false;
     bool parse_only = false;
     bool efdc_inputs = false;
+    // fault injection for tests: once this many outputs are out, the next
+    // step request asks for a negative iteration count, which the library
+    // rejects as it would any failed step (-1: off)
+    long fail_after_outputs = -1;
 };
 
 bool parse_args(int argc, char **argv, Options &o)
@@ -115,6 +119,7 @@ bool parse_args(int argc, char **argv, Options &o)
         else if (a == "--autotune") o.autotune = true;
         else if (a == "--parse-only") o.parse_only = true;
         else if (a == "--efdc-inputs") o.efdc_inputs = true;
+        else if (a == "--fail-step-after-outputs" && next(v)) o.fail_after_outputs = atol(v.c_str());
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
             return false;
@@ -356,6 +361,11 @@ int run_case(const Options &opt, const std::string &name)
     long out0 = -1;
     long pending = 0;
     std::string tempo = "0";
+    bool run_failed = false;   // a step or snapshot failed: the outputs stop there
+    auto step_request = [&](long n) {
+        const bool inject = opt.fail_after_outputs >= 0 && tq + 1 >= opt.fail_after_outputs;
+        return check(sw2d_step(h, inject ? -1 : (int)n), "step");
+    };
     const std::clock_t start = std::clock();
     while (time + dt <= timmax) {
         out0 = out0 + 1;
@@ -366,12 +376,18 @@ int run_case(const Options &opt, const std::string &name)
         time = time + dt;
         mstep = mstep + 1;
         if (!((mstep % lkout == 0) || (out0 == 0))) continue;
-        if (!check(sw2d_step(h, (int)pending), "step")) break;
+        if (!step_request(pending)) {
+            run_failed = true;
+            break;
+        }
         pending = 0;
         tq = tq + 1;
         tempo = (out0 > 0) ? tempo_name(round(mstep * dt / s.dkout)) : std::string("0");
         long ticket = -1;
-        if (fields && !check(sw2d_snapshot(h, fields, 3600.0, &ticket), "snapshot")) break;
+        if (fields && !check(sw2d_snapshot(h, fields, 3600.0, &ticket), "snapshot")) {
+            run_failed = true;
+            break;
+        }
         if (s.efdc == 1 && tq == 0 && !efdc_note) {
             // written with the first output, beside the EFDC.INP card update (4924-4938)
             const double *sv = &s.efdc_block[sw2dio::kNSTYPE];
@@ -426,9 +442,16 @@ int run_case(const Options &opt, const std::string &name)
             if (ticket >= 0) sw2d_snapshot_release(h, ticket);
         });
     }
-    if (pending > 0) check(sw2d_step(h, (int)pending), "step");
-    sw2d_sync(h);
+    if (!run_failed && pending > 0 && !step_request(pending)) run_failed = true;
+    if (sw2d_sync(h) != SW2D_OK) run_failed = true;
     writer.finish();
+    if (run_failed) {
+        printf(" the run stopped at t = %.6f s after a library error: %d output(s) written, "
+               "the rest of the run is missing\n", time, tq + 1);
+        fclose(wl);
+        sw2d_destroy(h);
+        return 1;
+    }
     // constituent series of the coupling records, after the run (5076-5092)
     if (s.efdc == 1 && tq >= 0) {
         const std::string in = c.dir + "/EFDC/input/";

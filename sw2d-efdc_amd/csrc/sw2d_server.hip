@@ -13,6 +13,7 @@
 // (their time per iteration is the tile's row chain), a full GPU as fewer,
 // taller ones (fewer halo rows re-read).
 #include <math.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <chrono>
@@ -70,6 +71,12 @@ struct sw2d_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     bool stop = false;
+    bool dead = false;         // a batch step failed: nothing more will run
+    // fault injection for tests (SW2D_FAULT_SERVER_STEP=k in the environment
+    // at create): slice k asks the batch for a negative iteration count,
+    // which sw2d_batch_step rejects like any failed step (-1: off)
+    long fault_slice = -1;
+    long slices = 0;
     std::deque<Request *> queue;
     std::vector<Request *> running;
     std::map<long, std::unique_ptr<Request>> all;
@@ -235,7 +242,7 @@ struct sw2d_server {
             }
             const double ts = now();
             int nrun = 0;
-            const int rc = sw2d_batch_step(batch, slice, &nrun);
+            const int rc = sw2d_batch_step(batch, slices++ == fault_slice ? -1 : slice, &nrun);
             {
                 double ms = 0;
                 long nl = 0;
@@ -250,14 +257,28 @@ struct sw2d_server {
                 pool_cap = cp;
             }
             if (rc) {
+                // the batch cannot be trusted any more: every request still
+                // running or queued fails with the error, and submit refuses
+                // new ones; waiters are woken
                 std::lock_guard<std::mutex> lk(mu);
-                err = sw2d_batch_last_error(batch);
+                err = std::string("batch step failed: ") + sw2d_batch_last_error(batch);
+                const double t = now();
                 for (Request *r : running) {
                     r->res.state = SW2D_REQ_FAILED;
+                    r->res.t_done = t;
                     r->err = err;
+                    active_cells -= cells_of(r);
                     failed++;
                 }
                 running.clear();
+                for (Request *r : queue) {
+                    r->res.state = SW2D_REQ_FAILED;
+                    r->res.t_done = t;
+                    r->err = err;
+                    failed++;
+                }
+                queue.clear();
+                dead = true;
                 cv_done.notify_all();
                 return;
             }
@@ -309,6 +330,7 @@ int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_ro
         return rc;
     }
     (void)sw2d_batch_timing(s->batch, 1);
+    if (const char *f = getenv("SW2D_FAULT_SERVER_STEP")) s->fault_slice = atol(f);
     s->t0 = std::chrono::steady_clock::now();
     s->thr = std::thread([s] { s->loop(); });
     *out = s;
@@ -328,6 +350,10 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = "bad request arrays or t_end";
         return SW2D_E_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->dead) return SW2D_E_STATE;   // err holds the step failure
     }
     auto r = std::make_unique<Request>();
     r->req = *q;

@@ -320,6 +320,27 @@ def test_driver_reports_an_output_write_failure(cases, tmp_path):
     assert (outdir / "Results_1.vtk").is_file()   # the other outputs were written
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("after", [0, 2])
+def test_driver_reports_a_step_failure(cases, tmp_path, after):
+    """A library error during the run (injected: once `after` outputs are
+    out, the next step request asks for a negative iteration count, which
+    sw2d_step rejects) ends the run with status 1 and a message; the outputs
+    taken before it are written, no later ones, and no TimeSimu file claims a
+    finished run."""
+    spec = small_case(cases)
+    base = pathlib.Path(cases.write_db(spec, tmp_path, "C6", **CASE_KW))
+    r = run_driver(tmp_path, "--fail-step-after-outputs", str(after), check=False)
+    assert r.returncode == 1, r.stdout[-2000:]
+    assert "step failed" in r.stdout and "the run stopped" in r.stdout
+    outdir = base / "SW2D" / "output"
+    assert len(list(outdir.glob("Results_*.vtk"))) == after
+    assert not list(outdir.glob("TimeSimu_*.txt"))
+    # the same case without the injection runs to the end
+    ok = run_driver(tmp_path)
+    assert ok.returncode == 0 and list(outdir.glob("TimeSimu_*.txt"))
+
+
 def vtk_block(text, name):
     """Values of one SCALARS block of a STRUCTURED_POINTS file (file order)."""
     lines = text.splitlines()

@@ -251,3 +251,29 @@ def test_fp64_server_with_adaptive_tiling_equals_alone(pkg, cases):
             assert r["balance"][k] == pytest.approx(bal[k], rel=1e-12, abs=1e-9), (i, k)
         c.close()
     srv.close()
+
+
+def test_step_failure_fails_every_request_and_stops_admission(pkg, cases, monkeypatch):
+    """A failed batch step (injected through SW2D_FAULT_SERVER_STEP: the
+    third slice asks the batch for a negative iteration count, which
+    sw2d_batch_step rejects) fails the running request and every queued one,
+    with their finish times set, so a waiter returns; the server then refuses
+    new requests instead of queueing work that would never run."""
+    monkeypatch.setenv("SW2D_FAULT_SERVER_STEP", "2")
+    reqs = [request(cases, i) for i in (0, 4, 5)]
+    srv = pkg.Server(precision="fp32", pool_bytes=256 << 20, tile_rows=H, slice_iters=12,
+                     max_active_cells=max(z.size for z, *_ in reqs) + 1)
+    monkeypatch.delenv("SW2D_FAULT_SERVER_STEP")
+    ids = [srv.submit(z, 10.0, t_end=20 * t_end, rain=rain, dtrain=900.0, outlet=outlet,
+                      hydro_cap=CAP, **kw) for z, outlet, rain, kw, t_end in reqs]
+    srv.wait(timeout=60)
+    res = [srv.poll(i) for i in ids]
+    assert [r["state"] for r in res] == ["failed"] * 3
+    assert all(r["t_done"] > 0 and r["t_done"] >= r["t_submit"] for r in res)
+    st = srv.stats()
+    assert st["failed"] == 3 and st["running"] == 0 and st["queued"] == 0
+    assert st["completed"] == 0
+    z, outlet, rain, kw, t_end = reqs[0]
+    with pytest.raises(pkg.sw2d.Sw2dError, match="batch step failed"):
+        srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet, **kw)
+    srv.close()
