@@ -41,6 +41,8 @@ extern "C" {
 /* arithmetic of the state and fluxes */
 #define SW2D_FP64 0   /* FP64 everywhere (parity mode) */
 #define SW2D_FP32 1   /* FP32 state/fluxes, FP64 dt, volume and hydrograph */
+#define SW2D_MIXED 2  /* FP32 fluxes and momentum arithmetic over FP64 depths: FP64 depth
+                         storage, continuity and depth switches, dt, volume, hydrograph */
 
 /* kernel organisation */
 #define SW2D_KERNEL_FUSED 0  /* one fused, row-streaming kernel per iteration */
@@ -437,7 +439,9 @@ typedef struct sw2d_server_stats {
  * shares it among more requests: the same throughput, later results. */
 int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_rows, int slice_iters,
                        long max_active_cells, sw2d_server **out);
-/* Copies the request's arrays; returns its id. */
+/* Copies the request's arrays; returns its id.  Once a batch step has failed
+ * the server stops: every running and queued request fails with the error
+ * (sw2d_server_last_error) and submit returns SW2D_E_STATE. */
 int sw2d_server_submit(sw2d_server *s, const sw2d_request *req, long *id);
 int sw2d_server_poll(sw2d_server *s, long id, sw2d_result *out);
 /* (t, q) pairs of a finished request: 2*min(n, cap) doubles. */

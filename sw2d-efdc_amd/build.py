@@ -21,7 +21,8 @@ DRIVER = os.path.join(HERE, "sw2d-efdc")
 # (source, extra flags); the FP32 kernel may fuse multiply-adds within one
 # source expression, nothing else may fuse any
 HIP_SOURCES = [("sw2d_api.hip", []), ("sw2d_fused64.hip", []),
-               ("sw2d_fused32.hip", ["-ffp-contract=on"]), ("sw2d_plain.hip", []),
+               ("sw2d_fused32.hip", ["-ffp-contract=on"]),
+               ("sw2d_fusedmix.hip", ["-ffp-contract=on"]), ("sw2d_plain.hip", []),
                ("sw2d_batch.hip", []), ("sw2d_server.hip", []), ("sw2d_output.hip", []),
                ("sw2d_host.cpp", [])]
 DRIVER_SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp"]

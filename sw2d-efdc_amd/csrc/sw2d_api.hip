@@ -96,9 +96,8 @@ int upload_desc(sw2d_handle *h)
 
 hipError_t launch_one(sw2d_handle *h, const Scenario *scen, int phase)
 {
-    return h->esize == 8
-               ? launch_fused_f64(h->stream, scen, nullptr, fused_blocks(h), phase, h->features)
-               : launch_fused_f32(h->stream, scen, nullptr, fused_blocks(h), phase, h->features);
+    return launch_fused(h->prm.precision, h->stream, scen, nullptr, fused_blocks(h), phase,
+                        h->features);
 }
 
 // the graph of iterations k .. k+5 (k % 6 == 0): six launches, phases 0..5
@@ -187,9 +186,9 @@ int flush_events(sw2d_handle *h)
 
 // Resident one-wave blocks of the fused kernel over the device (waves per CU
 // at its register use x CUs): one "round" of a launch.
-long wave_slots(int device, int esize, int features)
+long wave_slots(int device, int prec, int features)
 {
-    const int per_cu = esize == 8 ? fused_waves_per_cu_f64(features) : fused_waves_per_cu_f32(features);
+    const int per_cu = fused_waves_per_cu(prec, features);
     hipDeviceProp_t prop;
     int cus = 0;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) cus = prop.multiProcessorCount;
@@ -231,7 +230,8 @@ std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &
 int choose_tile_rows(const sw2d_handle *h)
 {
     if (h->prm.tile_rows > 0) return std::min(h->prm.tile_rows, 512);
-    const long slots = wave_slots(h->device, h->esize, features_of(make_common(const_cast<sw2d_handle *>(h))));
+    const long slots = wave_slots(h->device, h->prm.precision,
+                                  features_of(make_common(const_cast<sw2d_handle *>(h))));
     const std::vector<int> fit = round_fill_heights(slots, [&](int H) { return (long)blocks_at(h, H); });
     for (auto it = fit.rbegin(); it != fit.rend(); ++it)
         if (*it <= 48) return *it;
@@ -302,8 +302,7 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     if (!grid || !z || !h0 || !params) return fail(nullptr, SW2D_E_ARG, "NULL argument");
     if (grid->rows < 2 || grid->cols < 2) return fail(nullptr, SW2D_E_ARG, "grid must be at least 2x2");
     if (!(grid->cellsize > 0)) return fail(nullptr, SW2D_E_ARG, "cellsize must be positive");
-    if (params->precision != SW2D_FP64 && params->precision != SW2D_FP32)
-        return fail(nullptr, SW2D_E_ARG, "unknown precision");
+    if (!known_precision(params->precision)) return fail(nullptr, SW2D_E_ARG, "unknown precision");
     if (params->kernel != SW2D_KERNEL_FUSED && params->kernel != SW2D_KERNEL_PLAIN)
         return fail(nullptr, SW2D_E_ARG, "unknown kernel");
     if (params->kernel == SW2D_KERNEL_PLAIN && params->precision != SW2D_FP64)
@@ -340,7 +339,6 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     h->R = grid->rows; h->C = grid->cols;
     h->device = batch ? batch->device : params->device;
     h->prm.device = h->device;
-    h->esize = params->precision == SW2D_FP64 ? 8 : 4;
     int rc = SW2D_OK;
     auto bail = [&](int code) {
         g_create_error = h->err;
@@ -372,9 +370,13 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
         h->W = 58;
         set_tiles(h, (batch && batch->H > 0 && params->tile_rows == 0) ? batch->H : choose_tile_rows(h));
         h->nparts = h->ntiles;
+        // depths in the form's depth type, fluxes in its flux type
+        const int hb = depth_bytes(params->precision), fb = flux_bytes(params->precision);
         void **bufs[] = {&h->h[0], &h->h[1], &h->um[0], &h->um[1], &h->vn[0], &h->vn[1]};
-        for (void **b : bufs) {
-            if (h->esize == 8) { if ((rc = dalloc(h, (double **)b, n))) return bail(rc); }
+        for (int i = 0; i < 6; i++) {
+            void **b = bufs[i];
+            const int eb = i < 2 ? hb : fb;
+            if (eb == 8) { if ((rc = dalloc(h, (double **)b, n))) return bail(rc); }
             else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
         }
         if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
@@ -387,7 +389,7 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
                 zi[(size_t)r * h->P + j] = (zv != nodata) ? zv : NAN;
             }
         e = hipMemcpy(h->z, zi.data(), n * 8, hipMemcpyHostToDevice);
-        if (h->esize == 8) {
+        if (hb == 8) {
             std::vector<double> hi(n, 0.0);
             for (int r = 0; r < R; r++)
                 for (int j = 0; j < C; j++) hi[(size_t)r * h->P + j] = h0[(size_t)r * C + j];
@@ -431,7 +433,7 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     h->host_mstep = 0;
     h->step_index = 0;
 
-    e = !fused(h) ? launch_plain_init(h) : (h->esize == 8 ? launch_init_f64(h) : launch_init_f32(h));
+    e = !fused(h) ? launch_plain_init(h) : launch_init(h);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     if (e != hipSuccess) { hip_fail(h, e, "init kernel"); return bail(SW2D_E_HIP); }
     if (batch) batch->owned.push_back(h);
@@ -658,7 +660,7 @@ namespace sw2d {
 // the heights that fill whole rounds of wave slots on this grid
 static std::vector<int> fill_heights(const sw2d_handle *h)
 {
-    return round_fill_heights(wave_slots(h->device, h->esize, h->features),
+    return round_fill_heights(wave_slots(h->device, h->prm.precision, h->features),
                               [&](int H) { return (long)blocks_at(h, H); });
 }
 
@@ -955,8 +957,7 @@ int sw2d_get_field(sw2d_handle *h, int field, double *out)
         // derived fields go through an R*C FP64 buffer made on first use
         int rc;
         if (!h->scratch && (rc = dalloc(h, &h->scratch, RC))) return rc;
-        hipError_t e = h->esize == 8 ? launch_derive_f64(h, field, h->scratch, 0, (long)RC)
-                                     : launch_derive_f32(h, field, h->scratch, 0, (long)RC);
+        hipError_t e = launch_derive(h, field, h->scratch, 0, (long)RC);
         if (e != hipSuccess) return hip_fail(h, e, "derive");
         HIPCHK(h, hipMemcpyAsync(out, h->scratch, RC * 8, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1078,7 +1079,10 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
     if (check_idle(h) || !bytes) return SW2D_E_ARG;
     long long b = 0;
     const size_t RC = (size_t)h->R * h->C;
-    if (fused(h)) b += (long long)6 * h->R * h->P * h->esize + (long long)h->R * h->P * 8 + (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? (long long)h->R * h->P : 0);
+    const long long RP = (long long)h->R * h->P;
+    if (fused(h))
+        b += 2 * RP * (depth_bytes(h->prm.precision) + 2 * flux_bytes(h->prm.precision)) + RP * 8 +
+             (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? RP : 0);
     else b += (long long)16 * h->NA * 8;
     b += (long long)2 * h->nparts * sizeof(Partial);
     *bytes = b;

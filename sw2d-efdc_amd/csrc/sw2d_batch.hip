@@ -46,8 +46,7 @@ void bdrop_graph(sw2d_batch *b)
 hipError_t blaunch(sw2d_batch *b, const Scenario *scen, const TileRef *tiles, int nblocks, int phase,
                    int features)
 {
-    return b->esize == 8 ? launch_fused_f64(b->stream, scen, tiles, nblocks, phase, features)
-                         : launch_fused_f32(b->stream, scen, tiles, nblocks, phase, features);
+    return launch_fused(b->precision, b->stream, scen, tiles, nblocks, phase, features);
 }
 
 // descriptors and tile table of the attached members
@@ -172,7 +171,7 @@ int batch_fit_tiles(sw2d_batch *b, int *tile_rows, int *retiled)
         feats |= features_of(make_common(h));
     }
     if (!run.empty()) {
-        const long slots = wave_slots(b->device, b->esize, feats);
+        const long slots = wave_slots(b->device, b->precision, feats);
         const std::vector<int> fit = round_fill_heights(slots, [&](int H) {
             long n = 0;
             for (const sw2d_handle *h : run) n += blocks_at(h, h->prm.tile_rows > 0 ? h->prm.tile_rows : H);
@@ -202,7 +201,7 @@ int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_row
 {
     if (!out) return bfail(nullptr, SW2D_E_ARG, "out is NULL");
     *out = nullptr;
-    if (precision != SW2D_FP64 && precision != SW2D_FP32) return bfail(nullptr, SW2D_E_ARG, "unknown precision");
+    if (!known_precision(precision)) return bfail(nullptr, SW2D_E_ARG, "unknown precision");
     if (tile_rows < 0 || tile_rows > 512) return bfail(nullptr, SW2D_E_ARG, "tile_rows must be in [0, 512]");
     if (pool_bytes < (1u << 20)) return bfail(nullptr, SW2D_E_ARG, "pool must be at least 1 MiB");
     int ndev = 0;
@@ -211,7 +210,6 @@ int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_row
     sw2d_batch *b = new sw2d_batch();
     b->device = device;
     b->precision = precision;
-    b->esize = precision == SW2D_FP64 ? 8 : 4;
     b->H = tile_rows > 0 ? tile_rows : 32;
     auto bail = [&](hipError_t e, const char *what) {
         bhip(b, e, what);
@@ -267,11 +265,11 @@ size_t sw2d_batch_footprint(const sw2d_grid_desc *grid, const sw2d_params *param
     if (!grid || !params || grid->rows < 2 || grid->cols < 2) return 0;
     const size_t R = grid->rows, C = grid->cols;
     const size_t P = (size_t)round_up((int)C, 16);
-    const size_t es = params->precision == SW2D_FP64 ? 8 : 4;
+    const size_t hb = depth_bytes(params->precision), fb = flux_bytes(params->precision);
     const int H = params->tile_rows > 0 ? params->tile_rows : (tile_rows > 0 ? tile_rows : 32);
     const size_t ntiles = ((C + 57) / 58) * ((R + H - 1) / H);
     size_t b = 0;
-    b += 6 * pool_bytes_of(R * P * es);                         // h, um, vn ping-pong
+    b += 2 * pool_bytes_of(R * P * hb) + 4 * pool_bytes_of(R * P * fb);  // h, um, vn ping-pong
     b += pool_bytes_of(R * P * 8);                               // z
     b += pool_bytes_of(sizeof(Scenario));                        // descriptor
     b += pool_bytes_of(3 * sizeof(Clock));
@@ -399,7 +397,7 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     // heights whose launch over all members fills whole rounds of wave slots
     int feats_all = 0;
     for (sw2d_handle *h : b->members) feats_all |= features_of(make_common(h));
-    const long slots = wave_slots(b->device, b->esize, feats_all);
+    const long slots = wave_slots(b->device, b->precision, feats_all);
     for (int H : round_fill_heights(slots, [&](int H) {
              long n = 0;
              for (const sw2d_handle *h : b->members)

@@ -114,24 +114,27 @@ __device__ __forceinline__ double bc_rate(int kind, double h, double uua, double
 
 // Face velocities of a state held in a pitched (rows x pitch) layout, with the
 // reference's edge rules (hm_hn 362-398, uu1_vv1 404-452): every face outside
-// the grid, on row 0 and on column 0 carries zero flux.
-template <typename T>
+// the grid, on row 0 and on column 0 carries zero flux.  T is the type of the
+// fluxes, TH that of the depths (the mixed form keeps depths in FP64): the
+// face depth and its dry test are taken in TH, the velocity in T.
+template <typename T, typename TH = T>
 struct PitchedState {
-    const T *h, *um, *vn;
+    const TH *h;
+    const T *um, *vn;
     int R, C, P;
-    T th;
-    __device__ __forceinline__ T hv(int r, int j) const { return h[(long)r * P + j]; }
+    TH th;
+    __device__ __forceinline__ TH hv(int r, int j) const { return h[(long)r * P + j]; }
     __device__ __forceinline__ T uu1(int r, int j) const
     {
         if (r <= 0 || r >= R || j < 0 || j >= C) return T(0);
-        T hm = T(0.50) * (hv(r, j) + hv(r - 1, j));
-        return (hm >= th) ? um[(long)r * P + j] / hm : T(0.00);
+        TH hm = TH(0.50) * (hv(r, j) + hv(r - 1, j));
+        return (hm >= th) ? um[(long)r * P + j] / (T)hm : T(0.00);
     }
     __device__ __forceinline__ T vv1(int r, int j) const
     {
         if (r < 0 || r >= R || j <= 0 || j >= C) return T(0);
-        T hn = T(0.50) * (hv(r, j) + hv(r, j - 1));
-        return (hn >= th) ? vn[(long)r * P + j] / hn : T(0.0);
+        TH hn = TH(0.50) * (hv(r, j) + hv(r, j - 1));
+        return (hn >= th) ? vn[(long)r * P + j] / (T)hn : T(0.0);
     }
     // uua_vva, SW2D-EFDC.cu:489-522
     __device__ __forceinline__ T uua(int r, int j) const

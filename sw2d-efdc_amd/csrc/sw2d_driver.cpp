@@ -12,7 +12,7 @@
 // DYE/SALT/TEMP.INP, DSER/TSER/SSER).  EFDC.INP card patching, GEFDC/Gorp and
 // the mpirun of the Fortran lake model are not part of this build.
 //
-// usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32]
+// usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32|mixed]
 //                  [--device N] [--autotune] [--parse-only] [--efdc-inputs]
 // --efdc-inputs writes the EFDC inputs known before the run (WSER, ASER,
 // SHOW, DYE/SALT/TEMP.INP) and stops; no GPU is used.
@@ -114,7 +114,8 @@ bool parse_args(int argc, char **argv, Options &o)
         std::string v;
         if (a == "--root" && next(v)) o.root = v;
         else if (a == "--kernel" && next(v)) o.kernel = (v == "plain") ? SW2D_KERNEL_PLAIN : SW2D_KERNEL_FUSED;
-        else if (a == "--precision" && next(v)) o.precision = (v == "fp32") ? SW2D_FP32 : SW2D_FP64;
+        else if (a == "--precision" && next(v))
+            o.precision = (v == "fp32") ? SW2D_FP32 : (v == "mixed") ? SW2D_MIXED : SW2D_FP64;
         else if (a == "--device" && next(v)) o.device = atoi(v.c_str());
         else if (a == "--autotune") o.autotune = true;
         else if (a == "--parse-only") o.parse_only = true;

@@ -11,13 +11,13 @@ namespace sw2d {
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
                            int phase, int features)
 {
-    return launch_fused_impl<float>(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_impl<float, float>(stream, scen, tiles, nblocks, phase, features);
 }
-int fused_waves_per_cu_f32(int features) { return fused_waves_per_cu_impl<float>(features); }
-hipError_t launch_init_f32(sw2d_handle *h) { return launch_init_impl<float>(h); }
+int fused_waves_per_cu_f32(int features) { return fused_waves_per_cu_impl<float, float>(features); }
+hipError_t launch_init_f32(sw2d_handle *h) { return launch_init_impl<float, float>(h); }
 hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out, long first, long count)
 {
-    return launch_derive_impl<float>(h, field, out, first, count);
+    return launch_derive_impl<float, float>(h, field, out, first, count);
 }
 
 }  // namespace sw2d

@@ -13,13 +13,13 @@ __global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *ou
 hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
                            int phase, int features)
 {
-    return launch_fused_impl<double>(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_impl<double, double>(stream, scen, tiles, nblocks, phase, features);
 }
-int fused_waves_per_cu_f64(int features) { return fused_waves_per_cu_impl<double>(features); }
-hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double>(h); }
+int fused_waves_per_cu_f64(int features) { return fused_waves_per_cu_impl<double, double>(features); }
+hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double, double>(h); }
 hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out, long first, long count)
 {
-    return launch_derive_impl<double>(h, field, out, first, count);
+    return launch_derive_impl<double, double>(h, field, out, first, count);
 }
 
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc)

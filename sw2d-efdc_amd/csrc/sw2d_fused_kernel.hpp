@@ -25,10 +25,15 @@
 // DPP lane moves.  No LDS staging and no barriers: each wave is independent.
 // The grid is one wave per tile plus the bookkeeping waves.
 //
-// Instantiated twice: sw2d_fused64.hip (T = double, built with
+// T is the type of the fluxes and of the momentum arithmetic, TH that of the
+// depths, the continuity updates and every switch that tests a depth.
+// Instantiated three times: sw2d_fused64.hip (T = TH = double, built with
 // -ffp-contract=off: every FP64 expression is written in the reference's
 // evaluation order, so the results equal the reference arithmetic as restated
-// by the CPU FP64 check, bit for bit) and sw2d_fused32.hip (T = float).
+// by the CPU FP64 check, bit for bit), sw2d_fused32.hip (T = TH = float) and
+// sw2d_fusedmix.hip (T = float, TH = double: FP32 fluxes and momentum
+// arithmetic over FP64 depths, so the dry-bed and threshold switches see the
+// depths the FP64 form sees).
 #pragma once
 #include "sw2d_common.hpp"
 
@@ -49,12 +54,13 @@ constexpr int kMaxTileRows = 512;
 // what remains is the same in every instantiation (FP64 fuses no
 // multiply-add, FP32 only within one source expression), so a scenario's
 // results do not depend on the instantiation that ran it.
-template <typename T, int F>
+template <typename T, typename TH, int F>
 __global__ void __launch_bounds__(64)
 fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles, const int nblocks,
            const int phase)
 {
     using M = num<T>;
+    using MH = num<TH>;
     constexpr bool kStorm = F & kFeatStorm, kSources = F & kFeatSources;
     constexpr bool kAdaptive = F & kFeatAdaptive, kMap = F & kFeatMap;
     const int lane = threadIdx.x;
@@ -99,18 +105,18 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // the descriptor's pointers are global memory: saying so lets the
     // compiler address them as SGPR base + VGPR offset (global_load) instead
     // of 64-bit flat addresses in VGPRs
-    const gptr<const T> h_in = (gptr<const T>)d.st[ib][0];
+    const gptr<const TH> h_in = (gptr<const TH>)d.st[ib][0];
     const gptr<const T> um_in = (gptr<const T>)d.st[ib][1];
     const gptr<const T> vn_in = (gptr<const T>)d.st[ib][2];
-    const gptr<T> h_out = (gptr<T>)d.st[ib ^ 1][0];
+    const gptr<TH> h_out = (gptr<TH>)d.st[ib ^ 1][0];
     const gptr<T> um_out = (gptr<T>)d.st[ib ^ 1][1];
     const gptr<T> vn_out = (gptr<T>)d.st[ib ^ 1][2];
     const gptr<const double> zg = (gptr<const double>)d.z;
     const gptr<const uint8_t> clsg = (gptr<const uint8_t>)d.cls;
 
     if (tile >= d.ntiles) {
-        PitchedState<T> ps{(const T *)d.st[ib][0], (const T *)d.st[ib][1], (const T *)d.st[ib][2],
-                           s.R, s.C, d.P, (T)s.th};
+        PitchedState<T, TH> ps{(const TH *)d.st[ib][0], (const T *)d.st[ib][1],
+                               (const T *)d.st[ib][2], s.R, s.C, d.P, (TH)s.th};
         auto vel = [&](int r, int j, double &hh, double &ua, double &va) {
             hh = (double)ps.hv(r, j);
             ua = (double)ps.uua(r, j);
@@ -144,12 +150,16 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const bool adapt = kAdaptive && !s.fixed_dt;
     const bool storm_on = kStorm && s.storm_on;
     const bool map_on = kMap && s.ncls_on;
-    // FP32 form: water-surface tests from bed differences carried per face
-    // (z stays FP64 only to form them), storm factors and source terms in FP32
+    // FP32 arithmetic: storm factors and source terms in FP32.  FP32 depths:
+    // water-surface tests from bed differences carried per face (z stays
+    // FP64 only to form them) and stochastically rounded depth updates.
     constexpr bool kF32 = sizeof(T) == 4;
+    constexpr bool kH32 = sizeof(TH) == 4;
     const T th = (T)s.th, gg = (T)s.g, dx = (T)s.dx, dy = (T)s.dy;
+    const TH thh = (TH)s.th;
     const T rdx = (T)(1.0 / s.dx), rdy = (T)(1.0 / s.dy);
     const T dt2 = (T)st.dt2;
+    const TH dt2h = (TH)st.dt2;
     const T dt2dx = (T)(st.dt2 / s.dx), dt2dy = (T)(st.dt2 / s.dy);
     const T rn = (T)s.rn;
 
@@ -175,20 +185,22 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         __syncthreads();
     }
     const T rr1t = (T)rr1, rr2t = (T)rr2, ev1t = (T)ev1, ev2t = (T)ev2;
-    const T P_lake = (T)s.P_lake, P_land = (T)s.P_land, ewl = (T)s.EV_WL_min;
+    const T P_lake = (T)s.P_lake, P_land = (T)s.P_land;
+    const TH ewl = (TH)s.EV_WL_min;
 
     // outlet
     const bool is_out_col = s.out_edge && (j == s.out_c);
     T qo_eff = 0;
 
     // carried rows: suffix 1 = row k-1, 2 = k-2, 3 = k-3
-    T h1 = 0, h2 = 0, h3 = 0, h1w = 0;
+    TH h1 = 0, h2 = 0, h3 = 0, h1w = 0;
     double z1 = 0, z2 = 0, z1w = 0;
     bool a1 = false, a2 = false, a3 = false, a1w = false;
     T u1 = 0, u2 = 0, u1e = 0, u1w = 0, v1 = 0, v2 = 0, v1e = 0, v1w = 0;
     T uu1_1 = 0, uu1_2 = 0, uu1_1w = 0;
     T vv1_1 = 0, vv1_2 = 0, vv1_1e = 0, vv1_2e = 0, vv1_1w = 0;
     T hm1 = 0, hn1 = 0, vva1 = 0;
+    bool fr_x1 = false, fr_y1 = false;  // friction on: face depth above th (taken in TH)
     T dzn1 = 0, dzw1 = 0;  // FP32 form: z(k-2) - z(k-1) and z(k-1, j-1) - z(k-1, j)
     T nc1 = rn, nc2 = rn, nc1w = rn;
     T umnP = 0, vnnP = 0, umpP = 0, vnpP = 0;
@@ -204,25 +216,28 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // values (0, inactive), so the loads need no branch.
     const int kbeg = r0 - 3, kend = r1 + 3;
     const int jc = colin ? j : (j < 0 ? 0 : C - 1);
-    auto load_row = [&](int k, T &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
+    auto load_row = [&](int k, TH &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
         const int off = kc * P + jc;
-        const T lh = h_in[off], lu = um_in[off], lv = vn_in[off];
+        const TH lh = h_in[off];
+        const T lu = um_in[off], lv = vn_in[off];
         const double lz = zg[off];
-        hh = in ? lh : T(0);
+        hh = in ? lh : TH(0);
         uu = in ? lu : T(0);
         vv = in ? lv : T(0);
         zz = in ? lz : __builtin_nan("");
         cc = (map_on && in) ? clsg[off] : (uint8_t)0;
     };
-    T nh, nu, nv;
+    TH nh;
+    T nu, nv;
     double nz;
     uint8_t ncl;
     load_row(kbeg, nh, nu, nv, nz, ncl);
 
     for (int k = kbeg; k < kend; ++k) {
-        const T h0 = nh, u0 = nu, v0 = nv;
+        const TH h0 = nh;
+        const T u0 = nu, v0 = nv;
         const double z0 = nz;
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
@@ -230,21 +245,22 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const T nc0 = map_on ? (T)s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
-        const T h0w = from_west(h0);
+        const TH h0w = from_west(h0);
         const double z0w = from_west(z0);
         const bool a0w = west_bit(a0, lane);
         const T u0e = from_east(u0), u0w = from_west(u0);
         const T v0e = from_east(v0), v0w = from_west(v0);
         const T nc0w = from_west(nc0);
         T dzn0 = 0, dzw0 = 0;
-        if constexpr (kF32) {
+        if constexpr (kH32) {
             dzn0 = (T)(z1 - z0);   // bed step across the north face of row k
             dzw0 = (T)(z0w - z0);  // bed step across the west face of row k
         }
-        const T hn0 = (j == 0) ? h0 : T(0.50) * (h0 + h0w);
-        const T hm0 = (k == 0) ? h0 : ((k == R) ? h1 : T(0.50) * (h0 + h1));
-        const T uu1_0 = (hm0 >= th) ? M::div_(u0, hm0) : T(0.00);
-        const T vv1_0 = (hn0 >= th) ? M::div_(v0, hn0) : T(0.0);
+        const TH hn0d = (j == 0) ? h0 : TH(0.50) * (h0 + h0w);
+        const TH hm0d = (k == 0) ? h0 : ((k == R) ? h1 : TH(0.50) * (h0 + h1));
+        const T hn0 = (T)hn0d, hm0 = (T)hm0d;
+        const T uu1_0 = (hm0d >= thh) ? M::div_(u0, hm0) : T(0.00);
+        const T vv1_0 = (hn0d >= thh) ? M::div_(v0, hn0) : T(0.0);
         const T vv1_0e = from_east(vv1_0), vv1_0w = from_west(vv1_0);
         const T uu1_0w = from_west(uu1_0);
         // uua_vva (489-522): vva of row k, uua of row k-1
@@ -255,11 +271,11 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
 
         if (adapt && colout && k >= r0 && k < r1) {
             // CFL candidates of the incoming state on the faces of row k
-            if (k >= 1 && a0 && a1 && hm0 > th) {
+            if (k >= 1 && a0 && a1 && hm0d > thh) {
                 const T w = M::abs_(uu1_0) + M::sqrt_(gg * hm0);
                 smax_x = (w > smax_x) ? w : smax_x;
             }
-            if (j >= 1 && a0 && a0w && hn0 > th) {
+            if (j >= 1 && a0 && a0w && hn0d > thh) {
                 const T w = M::abs_(vv1_0) + M::sqrt_(gg * hn0);
                 smax_y = (w > smax_y) ? w : smax_y;
             }
@@ -273,13 +289,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // x direction: um at the north face of (f, j); w = row f-1, e = row f.
         // Dry-bed branches B1-B3 all read (coef * hv) * sqrt(c2 * hv); each of
         // that and B4 (the momentum update) runs only when a lane needs it.
-        const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > th) || (h1 > th));
+        const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > thh) || (h1 > thh));
         if (__any(act_x)) {
             // water surfaces: FP64 (the reference's arithmetic) or, in the
             // FP32 form, bed difference dz = z2 - z1 plus FP32 depths
             bool b1, b2;
             T dsurf, hv3;  // (h1 + z1) - (h2 + z2); max(surfaces) - max(beds)
-            if constexpr (kF32) {
+            if constexpr (kH32) {
                 const T dz = dzn1;
                 b1 = h1 < dz;
                 b2 = !b1 && (h2 < -dz);
@@ -292,14 +308,14 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 dsurf = (T)(hhe - h2 - z2);
                 hv3 = (T)(__builtin_fmax(hhe, hhw) - __builtin_fmax(z1, z2));
             }
-            const T hhep = h1 - th, hhwp = h2 - th;
+            const TH hhep = h1 - thh, hhwp = h2 - thh;
             const bool b3 = !b1 && !b2 && (hhep * hhwp < 0);
             const bool b4 = act_x && !b1 && !b2 && !b3;
-            T coef = T(0.5440), hv = h2, c2 = gg;
-            bool wet = h2 > th;
-            if (b2) { coef = T(-0.544); hv = h1; wet = h1 > th; }
+            T coef = T(0.5440), hv = (T)h2, c2 = gg;
+            bool wet = h2 > thh;
+            if (b2) { coef = T(-0.544); hv = (T)h1; wet = h1 > thh; }
             if (b3) {
-                const T sgnm = M::sign_(hhep - hhwp);  // hhan / |hhan|
+                const T sgnm = (T)MH::sign_(hhep - hhwp);  // hhan / |hhan|
                 coef = -sgnm * T(0.350);
                 hv = hv3;
                 c2 = T(2.00) * gg;
@@ -321,7 +337,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 const T u12 = (j != 0) ? dt2dy * (umu - umd) : T(0);
                 const T nf = map_on ? T(0.5) * (nc2 + nc1) : rn;
                 T ram = 0.00;
-                if (b4 && !(hm1 <= th)) {
+                if (b4 && fr_x1) {
                     const T sqx = M::sqrt_(uu1_1 * uu1_1 + vva1 * vva1);
                     ram = M::div_(gg * nf * nf * sqx, pow1333(hm1));
                 }
@@ -332,11 +348,11 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             umn = act_x ? val : T(0);
         }
         // y direction: vn at the west face of (f, j); s = column j-1, n = column j
-        const bool act_y = flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > th) || (h1 > th));
+        const bool act_y = flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > thh) || (h1 > thh));
         if (__any(act_y)) {
             bool b1, b2;
             T dsurf, hv3;  // (h1 + z1) - (h1w + z1w); max(surfaces) - max(beds)
-            if constexpr (kF32) {
+            if constexpr (kH32) {
                 const T dz = dzw1;
                 b1 = h1 < dz;
                 b2 = !b1 && (h1w < -dz);
@@ -349,14 +365,14 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 dsurf = (T)(hhn - h1w - z1w);
                 hv3 = (T)(__builtin_fmax(hhn, hhs) - __builtin_fmax(z1, z1w));
             }
-            const T hhnp = h1 - th, hhsp = h1w - th;
+            const TH hhnp = h1 - thh, hhsp = h1w - thh;
             const bool b3 = !b1 && !b2 && (hhnp * hhsp < 0);
             const bool b4 = act_y && !b1 && !b2 && !b3;
-            T coef = T(0.5440), hv = h1w, c2 = gg;
-            bool wet = h1w > th;
-            if (b2) { coef = T(-0.544); hv = h1; wet = h1 > th; }
+            T coef = T(0.5440), hv = (T)h1w, c2 = gg;
+            bool wet = h1w > thh;
+            if (b2) { coef = T(-0.544); hv = (T)h1; wet = h1 > thh; }
             if (b3) {
-                const T sgnm = M::sign_(hhnp - hhsp);
+                const T sgnm = (T)MH::sign_(hhnp - hhsp);
                 coef = -sgnm * T(0.350);
                 hv = hv3;
                 c2 = T(2.00) * gg;
@@ -378,7 +394,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 const T v12 = dt2dy * (vnu - vnd);
                 const T nf = map_on ? T(0.5) * (nc1w + nc1) : rn;
                 T ram = 0.00;
-                if (b4 && !(hn1 <= th)) {
+                if (b4 && fr_y1) {
                     const T sqy = M::sqrt_(uua1 * uua1 + vv1_1 * vv1_1);
                     ram = M::div_(gg * nf * nf * sqy, pow1333(hn1));
                 }
@@ -410,18 +426,20 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 if (s.src_r[i] == g && s.src_c[i] == j) ql = (T)source_rate(s, i, st.t);
             T ums = umn, umnn = umnP, vne = vnnPe, vnw = vnnP;
             if (is_out_col && g == s.out_r) {
-                const T qo = (h2 > th) ? T(0.5440) * h2 * M::sqrt_(gg * h2) : T(0);
+                const T h2t = (T)h2;
+                const T qo = (h2 > thh) ? T(0.5440) * h2t * M::sqrt_(gg * h2t) : T(0);
                 if (s.out_edge == 1) ums = qo;
                 else if (s.out_edge == 2) umnn = -qo;
                 else if (s.out_edge == 3) vnw = -qo;
                 else vne = qo;
                 qo_eff = qo;
             }
-            const T hr = h2 - dt2 * (M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) -
-                                     ql - rr * Pc + E);
-            T h1v = M::max_(hr, T(0.00));
+            const T rhs = M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) - ql -
+                          rr * Pc + E;
+            const TH hr = h2 - dt2h * (TH)rhs;
+            TH h1v = MH::max_(hr, TH(0.00));
             if (!a2) h1v = 0.00;
-            dryg = a2 && (h1v < th);
+            dryg = a2 && (h1v < thh);
             // ---- treat_error on the faces of row g (695-725) ----
             umpg = umnP;
             if (dryP && umpg > 0) umpg = 0;
@@ -460,9 +478,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             }
             const T rhs = M::divc_(ums - umnn, dx, rdx) + M::divc_(vne - vnw, dy, rdy) - ql -
                           rr * Pc + E;
-            T h2v;
+            TH h2v;
             double clip = 0.0;
-            if constexpr (kF32) {
+            if constexpr (kH32) {
                 // FP32 form: the depth update h3 - dt2 rhs is rounded
                 // stochastically.  TwoSum gives the exact rounding error er of
                 // sm = fl(h3 + tt); the result moves to the FP32 neighbour of sm
@@ -486,8 +504,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                 const double hrd = (double)sm + (double)er;
                 if (hrd < 0.0) { clip = -hrd; h2v = 0; }
             } else {
-                const T hr = h3 - dt2 * rhs;
-                h2v = M::max_(hr, T(0.00));
+                const TH hr = h3 - dt2h * (TH)rhs;
+                h2v = MH::max_(hr, TH(0.00));
                 if (hr < 0) clip = -(double)hr;
             }
             if (!a3) h2v = 0.00;
@@ -515,6 +533,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         uu1_2 = uu1_1; uu1_1 = uu1_0; uu1_1w = uu1_0w;
         vv1_2 = vv1_1; vv1_2e = vv1_1e; vv1_1 = vv1_0; vv1_1e = vv1_0e; vv1_1w = vv1_0w;
         hm1 = hm0; hn1 = hn0; vva1 = vva0;
+        fr_x1 = !(hm0d <= thh); fr_y1 = !(hn0d <= thh);
         dzn1 = dzn0; dzw1 = dzw0;
         nc2 = nc1; nc1 = nc0; nc1w = nc0w;
         umnP = umn; vnnP = vnn;
@@ -544,8 +563,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
 }
 
 // initial CFL minimum of the state (adaptive mode)
-template <typename T>
-__global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T *vn,
+template <typename T, typename TH>
+__global__ void __launch_bounds__(256) init_cfl(const TH *h, const T *um, const T *vn,
                                                 const double *z, int R, int C, int P, double dx,
                                                 double dy, double g, double th, Clock *clk)
 {
@@ -559,18 +578,18 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
         const bool ac = (z[o] == z[o]);
         if (r >= 1) {
             const bool an = (z[o - P] == z[o - P]);
-            const T hm = T(0.50) * (h[o] + h[o - P]);
-            if (ac && an && hm > (T)th) {
-                const T uu1 = (hm >= (T)th) ? M::div_(um[o], hm) : T(0.00);
+            const TH hm = TH(0.50) * (h[o] + h[o - P]);
+            if (ac && an && hm > (TH)th) {
+                const T uu1 = (hm >= (TH)th) ? M::div_(um[o], (T)hm) : T(0.00);
                 const double w = dx / ((double)M::abs_(uu1) + sqrt(g * (double)hm));
                 cfl = (w < cfl) ? w : cfl;
             }
         }
         if (j >= 1) {
             const bool aw = (z[o - 1] == z[o - 1]);
-            const T hn = T(0.50) * (h[o] + h[o - 1]);
-            if (ac && aw && hn > (T)th) {
-                const T vv1 = (hn >= (T)th) ? M::div_(vn[o], hn) : T(0.0);
+            const TH hn = TH(0.50) * (h[o] + h[o - 1]);
+            if (ac && aw && hn > (TH)th) {
+                const T vv1 = (hn >= (TH)th) ? M::div_(vn[o], (T)hn) : T(0.0);
                 const double w = dy / ((double)M::abs_(vv1) + sqrt(g * (double)hn));
                 cfl = (w < cfl) ? w : cfl;
             }
@@ -584,13 +603,13 @@ __global__ void __launch_bounds__(256) init_cfl(const T *h, const T *um, const T
 // one derived field of the current state (hm_hn, uu1_vv1, uu_vv, uua_vva) over
 // the cells first .. first + count - 1 of the row-major rows x cols order;
 // out[i] receives cell first + i
-template <typename T>
-__global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, const T *vn,
+template <typename T, typename TH>
+__global__ void __launch_bounds__(256) derive_field(const TH *h, const T *um, const T *vn,
                                                     const double *z, int R, int C, int P,
                                                     double th, int field, double *out,
                                                     long first, long count)
 {
-    PitchedState<T> ps{h, um, vn, R, C, P, (T)th};
+    PitchedState<T, TH> ps{h, um, vn, R, C, P, (TH)th};
     const long n = first + count;
     for (long idx = first + (long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
          idx += (long)gridDim.x * blockDim.x) {
@@ -601,12 +620,12 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
         case SW2D_FIELD_H: v = h[o]; break;
         case SW2D_FIELD_UM: v = um[o]; break;
         case SW2D_FIELD_VN: v = vn[o]; break;
-        case SW2D_FIELD_HM: v = (r == 0) ? h[o] : T(0.50) * (h[o] + h[o - P]); break;
-        case SW2D_FIELD_HN: v = (j == 0) ? h[o] : T(0.50) * (h[o] + h[o - 1]); break;
+        case SW2D_FIELD_HM: v = (r == 0) ? h[o] : TH(0.50) * (h[o] + h[o - P]); break;
+        case SW2D_FIELD_HN: v = (j == 0) ? h[o] : TH(0.50) * (h[o] + h[o - 1]); break;
         case SW2D_FIELD_UU1: v = ps.uu1(r, j); break;
         case SW2D_FIELD_VV1: v = ps.vv1(r, j); break;
-        case SW2D_FIELD_UU: v = (h[o] >= (T)th) ? (ps.uu1(r + 1, j) + ps.uu1(r, j)) / T(2.00) : T(0.00); break;
-        case SW2D_FIELD_VV: v = (h[o] >= (T)th) ? (ps.vv1(r, j + 1) + ps.vv1(r, j)) / T(2.00) : T(0.00); break;
+        case SW2D_FIELD_UU: v = (h[o] >= (TH)th) ? (ps.uu1(r + 1, j) + ps.uu1(r, j)) / T(2.00) : T(0.00); break;
+        case SW2D_FIELD_VV: v = (h[o] >= (TH)th) ? (ps.vv1(r, j + 1) + ps.vv1(r, j)) / T(2.00) : T(0.00); break;
         case SW2D_FIELD_UUA: v = ps.uua(r, j); break;
         case SW2D_FIELD_VVA: v = ps.vva(r, j); break;
         case SW2D_FIELD_Z: v = z[o]; break;
@@ -616,70 +635,70 @@ __global__ void __launch_bounds__(256) derive_field(const T *h, const T *um, con
     }
 }
 
-template <typename T>
+template <typename T, typename TH>
 static hipError_t launch_fused_impl(hipStream_t stream, const Scenario *scen, const TileRef *tiles,
                                     int nblocks, int phase, int features)
 {
     switch (fused_variant(features)) {
     case kVariant0:
-        hipLaunchKernelGGL((fused_step<T, kVariant0>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant0>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant1:
-        hipLaunchKernelGGL((fused_step<T, kVariant1>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant1>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant2:
-        hipLaunchKernelGGL((fused_step<T, kVariant2>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant2>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant3:
-        hipLaunchKernelGGL((fused_step<T, kVariant3>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant3>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
         break;
     default:
-        hipLaunchKernelGGL((fused_step<T, kFeatAll>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kFeatAll>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
         break;
     }
     return hipGetLastError();
 }
 
 // resident one-wave blocks per CU of the instantiation that runs `features`
-template <typename T>
+template <typename T, typename TH>
 static int fused_waves_per_cu_impl(int features)
 {
     const void *fn;
     switch (fused_variant(features)) {
-    case kVariant0: fn = (const void *)fused_step<T, kVariant0>; break;
-    case kVariant1: fn = (const void *)fused_step<T, kVariant1>; break;
-    case kVariant2: fn = (const void *)fused_step<T, kVariant2>; break;
-    case kVariant3: fn = (const void *)fused_step<T, kVariant3>; break;
-    default: fn = (const void *)fused_step<T, kFeatAll>; break;
+    case kVariant0: fn = (const void *)fused_step<T, TH, kVariant0>; break;
+    case kVariant1: fn = (const void *)fused_step<T, TH, kVariant1>; break;
+    case kVariant2: fn = (const void *)fused_step<T, TH, kVariant2>; break;
+    case kVariant3: fn = (const void *)fused_step<T, TH, kVariant3>; break;
+    default: fn = (const void *)fused_step<T, TH, kFeatAll>; break;
     }
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, 64, 0) != hipSuccess) return 0;
     return nb;
 }
 
-template <typename T>
+template <typename T, typename TH>
 static hipError_t launch_init_impl(sw2d_handle *h)
 {
     if (!h->prm.adaptive) return hipSuccess;
     const long n = (long)h->R * h->C;
     int blocks = (int)((n + 255) / 256);
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(init_cfl<T>, dim3(blocks), dim3(256), 0, h->stream,
-                       (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
+    hipLaunchKernelGGL((init_cfl<T, TH>), dim3(blocks), dim3(256), 0, h->stream,
+                       (const TH *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
                        (const double *)h->z, h->R, h->C, h->P, h->grid.cellsize, h->grid.cellsize,
                        h->prm.g, h->prm.th, h->clk);
     return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, typename TH>
 static hipError_t launch_derive_impl(sw2d_handle *h, int field, double *out_dev, long first,
                                      long count)
 {
     int blocks = (int)((count + 255) / 256);
     if (blocks > 8192) blocks = 8192;
     if (blocks < 1) return hipSuccess;
-    hipLaunchKernelGGL(derive_field<T>, dim3(blocks), dim3(256), 0, h->stream,
-                       (const T *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
+    hipLaunchKernelGGL((derive_field<T, TH>), dim3(blocks), dim3(256), 0, h->stream,
+                       (const TH *)h->h[h->cur], (const T *)h->um[h->cur], (const T *)h->vn[h->cur],
                        (const double *)h->z, h->R, h->C, h->P, h->prm.th, field, out_dev, first,
                        count);
     return hipGetLastError();

@@ -178,7 +178,6 @@ struct sw2d_handle {
     long retune_every = 0;              // sw2d_set_retune: iterations between retunings (0 = off)
     int retune_iters = 3;
     long long next_retune = 0;
-    int esize = 8;                      // bytes per state element
     // fused-path state (ping-pong)
     void *h[2] = {nullptr, nullptr}, *um[2] = {nullptr, nullptr}, *vn[2] = {nullptr, nullptr};
     void *z = nullptr;
@@ -243,7 +242,6 @@ struct sw2d_handle {
 struct sw2d_batch {
     int device = 0;
     int precision = SW2D_FP32;
-    int esize = 4;
     int features = 0;   // kFeat* over the attached members (OR)
     hipStream_t stream = nullptr;
     char *pool_base = nullptr;
@@ -280,15 +278,42 @@ hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const Tile
                            int phase, int features);
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
                            int phase, int features);
+hipError_t launch_fused_mix(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
+                            int phase, int features);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
+hipError_t launch_init_mix(sw2d_handle *h);
 // one derived field of cells first .. first + count - 1 (row-major, rows x
 // cols) into out_dev[0 .. count - 1]
 hipError_t launch_derive_f64(sw2d_handle *h, int field, double *out_dev, long first, long count);
 hipError_t launch_derive_f32(sw2d_handle *h, int field, double *out_dev, long first, long count);
+hipError_t launch_derive_mix(sw2d_handle *h, int field, double *out_dev, long first, long count);
 hipError_t launch_peek(sw2d_handle *h, const StepCommon &sc);
 int fused_waves_per_cu_f64(int features);  // resident one-wave blocks per CU
 int fused_waves_per_cu_f32(int features);
+int fused_waves_per_cu_mix(int features);
+
+// bytes of a stored depth and of a stored face flux in a precision form
+inline int depth_bytes(int prec) { return prec == SW2D_FP32 ? 4 : 8; }
+inline int flux_bytes(int prec) { return prec == SW2D_FP64 ? 8 : 4; }
+inline bool known_precision(int prec)
+{
+    return prec == SW2D_FP64 || prec == SW2D_FP32 || prec == SW2D_MIXED;
+}
+// the fused kernel of a precision form
+inline hipError_t launch_fused(int prec, hipStream_t stream, const Scenario *scen,
+                               const TileRef *tiles, int nblocks, int phase, int features)
+{
+    if (prec == SW2D_FP64) return launch_fused_f64(stream, scen, tiles, nblocks, phase, features);
+    if (prec == SW2D_FP32) return launch_fused_f32(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_mix(stream, scen, tiles, nblocks, phase, features);
+}
+inline int fused_waves_per_cu(int prec, int features)
+{
+    if (prec == SW2D_FP64) return fused_waves_per_cu_f64(features);
+    if (prec == SW2D_FP32) return fused_waves_per_cu_f32(features);
+    return fused_waves_per_cu_mix(features);
+}
 hipError_t launch_plain(sw2d_handle *h, const StepCommon &sc);
 hipError_t launch_plain_init(sw2d_handle *h);
 }  // namespace sw2d

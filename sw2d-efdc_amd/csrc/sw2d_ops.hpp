@@ -81,6 +81,21 @@ inline void changed(sw2d_handle *h)
     if (h->batch) h->batch->dirty = true;
 }
 
+// the fused kernel's initial CFL minimum and derived fields, in the
+// handle's precision form
+inline hipError_t launch_init(sw2d_handle *h)
+{
+    const int p = h->prm.precision;
+    return p == SW2D_FP64 ? launch_init_f64(h) : p == SW2D_FP32 ? launch_init_f32(h) : launch_init_mix(h);
+}
+inline hipError_t launch_derive(sw2d_handle *h, int field, double *out_dev, long first, long count)
+{
+    const int p = h->prm.precision;
+    if (p == SW2D_FP64) return launch_derive_f64(h, field, out_dev, first, count);
+    if (p == SW2D_FP32) return launch_derive_f32(h, field, out_dev, first, count);
+    return launch_derive_mix(h, field, out_dev, first, count);
+}
+
 // defined in sw2d_api.hip
 int round_up(int x, int m);
 StepCommon make_common(sw2d_handle *h);
@@ -88,7 +103,7 @@ bool fused(const sw2d_handle *h);
 int fused_blocks(const sw2d_handle *h);
 Scenario make_scenario(sw2d_handle *h);
 int choose_tile_rows(const sw2d_handle *h);
-long wave_slots(int device, int esize, int features);
+long wave_slots(int device, int prec, int features);
 int blocks_at(const sw2d_handle *h, int H);
 std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &blocks);
 void set_tiles(sw2d_handle *h, int H);

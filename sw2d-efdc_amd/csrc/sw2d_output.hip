@@ -146,8 +146,7 @@ int sw2d_snapshot(sw2d_handle *h, unsigned fields, double timeout_s, long *ticke
     for (int f = 0; f < kSnapFields && e == hipSuccess; f++) {
         if (!(fields >> f & 1u)) continue;
         if (fused(h))
-            e = h->esize == 8 ? launch_derive_f64(h, f, s->dev[f], 0, (long)RC)
-                              : launch_derive_f32(h, f, s->dev[f], 0, (long)RC);
+            e = launch_derive(h, f, s->dev[f], 0, (long)RC);
         else
             e = hipMemcpyAsync(s->dev[f], plain_field(h, f), RC * 8, hipMemcpyDeviceToDevice, h->stream);
     }
@@ -293,8 +292,7 @@ int sw2d_get_field_noalloc(sw2d_handle *h, int field, double *out)
             for (long r0 = 0; r0 < h->R && e == hipSuccess; r0 += rows_per) {
                 const long nr = std::min<long>(rows_per, h->R - r0);
                 const long first = r0 * h->C, count = nr * h->C;
-                e = h->esize == 8 ? launch_derive_f64(h, field, area->ptr, first, count)
-                                  : launch_derive_f32(h, field, area->ptr, first, count);
+                e = launch_derive(h, field, area->ptr, first, count);
                 if (e == hipSuccess)
                     e = hipMemcpyAsync(out + first, area->ptr, (size_t)count * 8,
                                        hipMemcpyDeviceToHost, h->stream);

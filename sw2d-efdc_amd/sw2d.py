@@ -14,7 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsw2d.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "sw2d.h")
 
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "mixed": MIXED}
 # scenario features the fused kernel compiles out where unused (sw2d_kernel_features)
 FEAT_STORM, FEAT_SOURCES, FEAT_ADAPTIVE, FEAT_MAP = 1, 2, 4, 8
 FUSED, PLAIN = 0, 1
@@ -257,7 +258,7 @@ def make_params(*, dt, manning, g=9.80, th=1.0e-4, INT=0.0, INF=0.0, LWL=0.0, EV
     lib().sw2d_default_params(ct.byref(p))
     p.g, p.manning, p.th, p.dt, p.time0 = g, manning, th, dt, time0
     p.INT, p.INF, p.LWL, p.EV_WL_min = INT, INF, LWL, EV_WL_min
-    p.precision = {"fp64": FP64, "fp32": FP32}[precision]
+    p.precision = PRECISIONS[precision]
     p.kernel = {"fused": FUSED, "plain": PLAIN}[kernel]
     p.adaptive = 1 if adaptive else 0
     p.courant = courant
@@ -494,7 +495,7 @@ class Batch:
         self._L = lib()
         self._b = _H()
         self._cases = []
-        rc = self._L.sw2d_batch_create(device, {"fp64": FP64, "fp32": FP32}[precision],
+        rc = self._L.sw2d_batch_create(device, PRECISIONS[precision],
                                        pool_bytes, tile_rows, ct.byref(self._b))
         if rc:
             raise Sw2dError(f"sw2d_batch_create failed ({rc}): "
@@ -571,7 +572,7 @@ class Server:
                  max_active_cells=0, device=0):
         self._L = lib()
         self._s = _H()
-        rc = self._L.sw2d_server_create(device, {"fp64": FP64, "fp32": FP32}[precision],
+        rc = self._L.sw2d_server_create(device, PRECISIONS[precision],
                                         pool_bytes, tile_rows, slice_iters, max_active_cells,
                                         ct.byref(self._s))
         if rc:

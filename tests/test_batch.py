@@ -57,7 +57,7 @@ def assert_same(a, b, tag):
                                  f"{x[tuple(d[0])]!r} vs {y[tuple(d[0])]!r}")
 
 
-@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64", "mixed"])
 def test_batch_equals_alone(pkg, cases, precision):
     H = 16
     iters = [37, 60, 25, 71, 48]

@@ -1,5 +1,6 @@
-"""Precision form on the pond-break case: FP32 state/fluxes with FP64 bed,
-time step, volume ledger and hydrograph, against the CPU FP64 oracle.
+"""Reduced-precision forms against FP64: the FP32 form (FP32 state/fluxes
+with FP64 bed, time step, volume ledger and hydrograph) and the mixed form
+(FP32 fluxes over FP64 depths), on the pond break and on ensemble scenarios.
 
 The case is the 4096^2 pond break at a size the oracle finishes in seconds:
 flat floodplain + fBm 0.5 m, dx = 2 m, a square reservoir held 4 m deep and
@@ -41,13 +42,16 @@ def run(gpu_or_oracle, t_end, sample=5.0):
     return arrival
 
 
-def test_pond_break_fp32_form(pkg, po, cases):
+@pytest.mark.parametrize("form", ["fp32", "mixed"])
+def test_pond_break_reduced_precision_forms(pkg, po, cases, form):
+    """The FP32 form and the mixed form (FP32 fluxes over FP64 depths)
+    against the FP64 oracle on the pond break."""
     spec, z, h0 = pond(cases)
     kw = dict(dt=spec.dt, manning=spec.manning, adaptive=True, courant=spec.courant)
     o = po.Oracle(z, h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
     o.set_adaptive(spec.courant, 2.0 * spec.dt)
     g64 = pkg.Case(z, h0, spec.cellsize, precision="fp64", **kw)
-    g32 = pkg.Case(z, h0, spec.cellsize, precision="fp32", **kw)
+    g32 = pkg.Case(z, h0, spec.cellsize, precision=form, **kw)
     a_o = run(o, spec.t_end)
     a_64 = run(g64, spec.t_end)
     a_32 = run(g32, spec.t_end)
@@ -64,7 +68,7 @@ def test_pond_break_fp32_form(pkg, po, cases):
     reached_o, reached_32 = int(np.isfinite(a_o).sum()), int(np.isfinite(a_32).sum())
     b = g32.balance()
     vol_err = abs(b["error"]) / b["volume"]
-    print(f"pond break {z.shape}: max|dh| {max_dh:.3e} m, L2 {l2:.3e} m, arrival error "
+    print(f"pond break {z.shape} {form}: max|dh| {max_dh:.3e} m, L2 {l2:.3e} m, arrival error "
           f"{arr_err:.1f} s (cells reached: oracle {reached_o}, fp32 {reached_32}), "
           f"volume error {vol_err:.2e}")
     assert max_dh <= 1e-3
@@ -139,4 +143,47 @@ def test_fp32_form_on_ensemble_scenarios(pkg, cases):
         assert np.sqrt(np.mean(dh ** 2)) <= 1e-4
         assert abs(q32 - q64) <= 1e-4 * q64
     b32.close()
+    b64.close()
+
+
+def test_mixed_form_on_ensemble_scenarios(pkg, cases):
+    """The mixed form (FP32 fluxes and momentum arithmetic over FP64 depths,
+    continuity and depth switches) against FP64 on the same ensemble
+    scenarios as the FP32 form, plus a 512^2 member.  Storing depths in FP64
+    removes the depth rounding that moves cells across the dry-bed switches
+    (FP32 form: 1.4-1.5 mm on the 200^2 and 256^2 members), and the volume
+    ledger closes without stochastic rounding.  The momentum update is still
+    FP32: where friction is stiff (shallow sheet flow) it is a difference of
+    near-equal terms, and on the 512^2 member the FP32 rounding of those
+    terms still moves a few cells by ~4 mm, as in the FP32 form.  Measured:
+    max|dh| 8e-5 / 6.7e-4 / 3.3e-4 / 3.8e-3 m, peak outflow within 1.3e-6,
+    volume error below 1e-8 (the flux divergence of each cell is FP32)."""
+    bm = pkg.Batch(precision="mixed", pool_bytes=512 << 20)
+    b64 = pkg.Batch(precision="fp64", pool_bytes=1 << 30)
+    pairs = []
+    for k, side in ((3, 128), (11, 200), (19, 256), (7, 512)):
+        spec = cases.ensemble_scenario(k, side=side)
+        ms = []
+        for b in (bm, b64):
+            m = b.case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+            m.set_rain(spec.rain, spec.dtrain)
+            m.set_outlet(*spec.outlet)
+            m.set_hydrograph(4000)
+            b.attach(m, t_end=spec.t_end)
+            ms.append(m)
+        pairs.append((side, ms))
+    while bm.step(120) + b64.step(120):
+        pass
+    bound = {128: 2e-4, 200: 1e-3, 256: 1e-3, 512: 5e-3}
+    for side, (mm, m64) in pairs:
+        dh = np.abs(mm.field("h") - m64.field("h"))
+        q, q64 = mm.balance()["q_peak"], m64.balance()["q_peak"]
+        b = mm.balance()
+        print(f"mixed {side}^2: max|dh| {dh.max():.2e} m, L2 {np.sqrt(np.mean(dh**2)):.2e} m, "
+              f"peak outflow {q:.5f} vs {q64:.5f}, volume error {b['error'] / b['volume']:.1e}")
+        assert dh.max() <= bound[side]
+        assert np.sqrt(np.mean(dh ** 2)) <= 2e-4
+        assert abs(q - q64) <= 1e-5 * q64
+        assert abs(b["error"]) <= 1e-7 * b["volume"]
+    bm.close()
     b64.close()

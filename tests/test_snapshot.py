@@ -26,8 +26,8 @@ def make(pkg, cases, **kw):
 
 
 @pytest.mark.parametrize("kw", [dict(precision="fp64"), dict(precision="fp32"),
-                                dict(precision="fp64", kernel="plain")],
-                         ids=["fp64", "fp32", "plain"])
+                                dict(precision="mixed"), dict(precision="fp64", kernel="plain")],
+                         ids=["fp64", "fp32", "mixed", "plain"])
 def test_snapshot_equals_get_field_at_its_step(pkg, cases, kw):
     a, b = make(pkg, cases, **kw), make(pkg, cases, **kw)
     a.step(20)
