@@ -552,3 +552,34 @@ def test_manning_normal_depth_on_plane(po):
     np.testing.assert_allclose(um[rows], hm[rows] ** (5.0 / 3.0) * math.sqrt(S) / n, rtol=0.03)
     led = o.ledger()
     assert led["q_out"] == pytest.approx(i * dx * dx * R, rel=1e-9)
+
+
+def test_scheme_result_is_discontinuous_in_the_bed(po, cases):
+    """The reference scheme switches discontinuously (a weir when a water
+    surface falls below the neighbour's bed, momentum otherwise; the dry
+    threshold), so on rough rained-on slopes its result is not a continuous
+    function of its inputs.  Ensemble member 67 (157^2, dx 10 m, 1 h block
+    storm): perturbing the bed by a relative 1e-15 moves no depth by more
+    than 1e-12 m, but 1 um of seeded noise on the bed (far below survey
+    accuracy) moves a pond's depth by several centimetres and the peak
+    outflow by 0.1 % or more: water takes another path.  Whether a given
+    perturbation flips a switch is a threshold event (some 10 nm draws do,
+    others do not).  This is the yardstick for reduced-precision forms:
+    rounding depths or fluxes to binary32 perturbs them by more than that."""
+    s = cases.ensemble_scenario(67)
+    iters = int(round(s.t_end / (2 * s.dt)))
+
+    def run(z):
+        o = po.Oracle(z, s.h0, s.cellsize, dt=s.dt, manning=s.manning, threads=4)
+        o.set_rain(s.rain, s.dtrain)
+        o.set_outlet(*s.outlet)
+        o.set_hydrograph(iters + 8)
+        o.step(iters)
+        return o.field("h"), o.hydrograph()[:, 1].max()
+
+    h0, q0 = run(s.z)
+    h1, q1 = run(s.z * (1.0 + 1e-15 * np.random.default_rng(1).standard_normal(s.z.shape)))
+    assert np.abs(h1 - h0).max() < 1e-12 and abs(q1 - q0) < 1e-12 * q0
+    h2, q2 = run(s.z + 1e-6 * np.random.default_rng(3).standard_normal(s.z.shape))
+    assert np.abs(h2 - h0).max() > 1e-2
+    assert abs(q2 - q0) > 1e-4 * q0
