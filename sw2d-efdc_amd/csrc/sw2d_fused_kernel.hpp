@@ -22,8 +22,11 @@
 //   the second continuity of row k-3              (continuity at t + dt)
 // and stores row k-3 of the new state.  Values of a row are carried in
 // registers while they are needed (7-row window); east/west neighbours are
-// DPP lane moves.  No LDS staging and no barriers: each wave is independent.
-// The grid is one wave per tile plus the bookkeeping waves.
+// DPP lane moves, made afresh in the row that uses them rather than carried
+// to the next one, and the face depths of row k-1 are derived where they are
+// used: register pressure sets the waves per SIMD.  No LDS staging and no
+// barriers: each wave is independent.  The grid is one wave per tile plus the
+// bookkeeping waves.
 //
 // T is the type of the fluxes and of the momentum arithmetic, TH that of the
 // depths, the continuity updates and every switch that tests a depth.
@@ -193,16 +196,15 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     T qo_eff = 0;
 
     // carried rows: suffix 1 = row k-1, 2 = k-2, 3 = k-3
-    TH h1 = 0, h2 = 0, h3 = 0, h1w = 0;
+    TH h1 = 0, h2 = 0, h3 = 0;
     double z1 = 0, z2 = 0, z1w = 0;
     bool a1 = false, a2 = false, a3 = false, a1w = false;
-    T u1 = 0, u2 = 0, u1e = 0, u1w = 0, v1 = 0, v2 = 0, v1e = 0, v1w = 0;
-    T uu1_1 = 0, uu1_2 = 0, uu1_1w = 0;
-    T vv1_1 = 0, vv1_2 = 0, vv1_1e = 0, vv1_2e = 0, vv1_1w = 0;
-    T hm1 = 0, hn1 = 0, vva1 = 0;
-    bool fr_x1 = false, fr_y1 = false;  // friction on: face depth above th (taken in TH)
+    T u1 = 0, u2 = 0, v1 = 0, v2 = 0;
+    T uu1_1 = 0, uu1_2 = 0;
+    T vv1_1 = 0, vv1_2 = 0;
+    T vva1 = 0;
     T dzn1 = 0, dzw1 = 0;  // FP32 form: z(k-2) - z(k-1) and z(k-1, j-1) - z(k-1, j)
-    T nc1 = rn, nc2 = rn, nc1w = rn;
+    T nc1 = rn, nc2 = rn;
     T umnP = 0, vnnP = 0, umpP = 0, vnpP = 0;
     bool dryP = false;
 
@@ -242,15 +244,20 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
         const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
+        // west/east neighbours of rows k-1 and k-2, made here rather than
+        // carried from the row before (shorter register live ranges)
+        const TH h1w = from_west(h1);
+        const T u1e = from_east(u1), u1w = from_west(u1);
+        const T v1e = from_east(v1), v1w = from_west(v1);
+        const T uu1_1w = from_west(uu1_1);
+        const T vv1_1e = from_east(vv1_1), vv1_2e = from_east(vv1_2), vv1_1w = from_west(vv1_1);
+        const T nc1w = map_on ? from_west(nc1) : rn;
         const T nc0 = map_on ? (T)s.ntab[cl0] : rn;
 
         // ---- row k: face depths and velocities (hm_hn, uu1_vv1) ----
         const TH h0w = from_west(h0);
         const double z0w = from_west(z0);
         const bool a0w = west_bit(a0, lane);
-        const T u0e = from_east(u0), u0w = from_west(u0);
-        const T v0e = from_east(v0), v0w = from_west(v0);
-        const T nc0w = from_west(nc0);
         T dzn0 = 0, dzw0 = 0;
         if constexpr (kH32) {
             dzn0 = (T)(z1 - z0);   // bed step across the north face of row k
@@ -291,6 +298,10 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // that and B4 (the momentum update) runs only when a lane needs it.
         const bool act_x = flux_um && f > 0 && f < R && a1 && a2 && ((h2 > thh) || (h1 > thh));
         if (__any(act_x)) {
+            // face depth of row k-1, derived as hm0 was for it
+            const TH hm1d = (k - 1 == 0) ? h1 : ((k - 1 == R) ? h2 : TH(0.50) * (h1 + h2));
+            const T hm1 = (T)hm1d;
+            const bool fr_x1 = !(hm1d <= thh);  // friction on: face depth above th
             // water surfaces: FP64 (the reference's arithmetic) or, in the
             // FP32 form, bed difference dz = z2 - z1 plus FP32 depths
             bool b1, b2;
@@ -350,6 +361,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         // y direction: vn at the west face of (f, j); s = column j-1, n = column j
         const bool act_y = flux_vn && j > 0 && j < C && a1 && a1w && ((h1w > thh) || (h1 > thh));
         if (__any(act_y)) {
+            const TH hn1d = (j == 0) ? h1 : TH(0.50) * (h1 + h1w);
+            const T hn1 = (T)hn1d;
+            const bool fr_y1 = !(hn1d <= thh);
             bool b1, b2;
             T dsurf, hv3;  // (h1 + z1) - (h1w + z1w); max(surfaces) - max(beds)
             if constexpr (kH32) {
@@ -525,17 +539,16 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         }
 
         // ---- rotate the row window ----
-        h3 = h2; h2 = h1; h1 = h0; h1w = h0w;
+        h3 = h2; h2 = h1; h1 = h0;
         z2 = z1; z1 = z0; z1w = z0w;
         a3 = a2; a2 = a1; a1 = a0; a1w = a0w;
-        u2 = u1; u1 = u0; u1e = u0e; u1w = u0w;
-        v2 = v1; v1 = v0; v1e = v0e; v1w = v0w;
-        uu1_2 = uu1_1; uu1_1 = uu1_0; uu1_1w = uu1_0w;
-        vv1_2 = vv1_1; vv1_2e = vv1_1e; vv1_1 = vv1_0; vv1_1e = vv1_0e; vv1_1w = vv1_0w;
-        hm1 = hm0; hn1 = hn0; vva1 = vva0;
-        fr_x1 = !(hm0d <= thh); fr_y1 = !(hn0d <= thh);
+        u2 = u1; u1 = u0;
+        v2 = v1; v1 = v0;
+        uu1_2 = uu1_1; uu1_1 = uu1_0;
+        vv1_2 = vv1_1; vv1_1 = vv1_0;
+        vva1 = vva0;
         dzn1 = dzn0; dzw1 = dzw0;
-        nc2 = nc1; nc1 = nc0; nc1w = nc0w;
+        nc2 = nc1; nc1 = nc0;
         umnP = umn; vnnP = vnn;
         dryP = dryg; umpP = umpg; vnpP = vnpg;
     }
