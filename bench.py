@@ -150,11 +150,18 @@ def batch_run(pkg, specs, precision, tile_rows=0, tune_after=300, dem_noise=0.0)
     b.timing(True)
     t0 = time.perf_counter()
     H = tile_rows
+    b.tuning = None
     if tile_rows == 0 and tune_after > 0:
-        # the tile height is tuned once the rain has wetted the slopes (on the
-        # dry initial state the flux work is skipped); trials are timed in
+        # the tile height and the launch schedule are tuned once the rain has
+        # wetted the slopes (on the dry initial state the flux work is
+        # skipped); trials are timed in
         b.step(tune_after)
         H = b.autotune(8)
+        order, waves = b.schedule()
+        b.tuning = {"candidates": [{"tile_rows": int(r[0]), "order": int(r[1]),
+                                    "waves_per_simd_cap": int(r[2]), "ms_per_launch": float(r[3])}
+                                   for r in b.tune_report()],
+                    "pick": {"tile_rows": H, "order": order, "waves_per_simd_cap": waves}}
     while b.step(60):
         pass
     wall = time.perf_counter() - t0
@@ -276,6 +283,7 @@ def batch_leg(pkg, cases, n, plain_sample):
     specs = [cases.ensemble_scenario(k) for k in range(n)]
     log(f"batch: {n} scenarios generated")
     b32, m32, wall, kms, launches, H = batch_run(pkg, specs, "fp32")
+    tuning = b32.tuning
     iters = [b32.status(m)[1] for m in m32]
     cell_updates = sum(s.cells * k for s, k in zip(specs, iters))
     bpc = bytes_per_cell("fp32")
@@ -316,6 +324,7 @@ def batch_leg(pkg, cases, n, plain_sample):
         "precision": "f32 state / f64 balance",
         "cells_total": int(sum(s.cells for s in specs)),
         "tile_rows": H,
+        "autotune": tuning,
         "iterations": int(max(iters)),
         "wall_s": wall,
         "value": cell_updates / wall,
@@ -425,7 +434,9 @@ def serve_leg(pkg, cases, seconds, rate, hours, precision, active_cells, tile, e
                 pick.append(i)
         pick = sorted(pick[:max(err_sample, len(set(sides)))])
         sub = [specs[i] for i in pick]
-        bq, mq, *_ = batch_run(pkg, sub, precision, tile_rows=32, tune_after=0)
+        # the served mix's sample as a batch, tuned as the ensemble is: the
+        # tuner's candidates and pick for this workload
+        bq, mq, *_ = batch_run(pkg, sub, precision)
         b64, m64, *_ = batch_run(pkg, sub, "fp64", tile_rows=32, tune_after=0)
         errs = member_errors(mq, m64)
         bq.close()
@@ -433,6 +444,7 @@ def serve_leg(pkg, cases, seconds, rate, hours, precision, active_cells, tile, e
         eno = member_errors(mno, m64)
         bno.close()
         b64.close()
+        out["autotune_on_sample"] = bq.tuning
         out["vs_fp64_sample"] = dict(spread_summary(errs, eno), requests=pick,
                                      sides=[sides[i] for i in pick],
                                      max_abs_dh_m=[e["max_abs_dh_m"] for e in errs],

@@ -363,6 +363,24 @@ int sw2d_batch_status(sw2d_batch *b, const sw2d_handle *h, int *done, long *iter
  * partial sums are folded, so the ledger is unchanged). */
 int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows);
 
+/* Launch schedule of the batch's step launches, the tuning dimensions beside
+ * the tile height: order 0 lays each member's blocks out in one run
+ * (neighbouring tiles share an XCD's L2), order 1 interleaves the members
+ * block by block (every stretch of a launch mixes scenarios, which evens out
+ * the work where some are wet and others dry); waves_per_simd > 0 caps the
+ * resident one-wave blocks per SIMD, and so the tiles and scenarios sharing
+ * a CU, by reserving unused LDS per block (0: the occupancy the registers
+ * allow).  Results do not depend on either.  sw2d_batch_autotune times both
+ * at its chosen tile height (orders 0 and 1, caps of one and two waves
+ * below the register limit) and keeps a change only when it is at least
+ * 1 % faster. */
+int sw2d_batch_set_schedule(sw2d_batch *b, int order, int waves_per_simd);
+int sw2d_batch_schedule(sw2d_batch *b, int *order, int *waves_per_simd);
+/* The trials of the last sw2d_batch_autotune, in the order run: rows of
+ * (tile rows, order, waves_per_simd, ms per launch), up to cap rows; *n
+ * receives the count. */
+int sw2d_batch_tune_report(sw2d_batch *b, double *rows, int cap, int *n);
+
 /* Pool use in bytes: now, the most so far, capacity. */
 int sw2d_batch_memory(sw2d_batch *b, size_t *in_use, size_t *peak, size_t *capacity);
 

@@ -46,7 +46,38 @@ void bdrop_graph(sw2d_batch *b)
 hipError_t blaunch(sw2d_batch *b, const Scenario *scen, const TileRef *tiles, int nblocks, int phase,
                    int features)
 {
-    return launch_fused(b->precision, b->stream, scen, tiles, nblocks, phase, features);
+    return launch_fused(b->precision, b->stream, scen, tiles, nblocks, phase, features, b->lds_pad);
+}
+
+// unused LDS per one-wave block that leaves room for at most `cap` resident
+// blocks per SIMD (four SIMDs per CU); 0 for no cap
+size_t lds_for_cap(int device, int cap)
+{
+    if (cap <= 0) return 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return 0;
+    size_t per = prop.maxSharedMemoryPerMultiProcessor / (4 * (size_t)cap);
+    per = per / 256 * 256;
+    return std::min<size_t>(per, prop.sharedMemPerBlock);
+}
+
+// the launch's tile table: each member's blocks in a run (order 0), or the
+// members interleaved block by block (order 1), so that every stretch of
+// the launch holds tiles of many members
+void order_tiles(const std::vector<int> &counts, const std::vector<int> &sids, int order,
+                 std::vector<TileRef> &tiles)
+{
+    tiles.clear();
+    if (order == 0) {
+        for (size_t m = 0; m < counts.size(); m++)
+            for (int i = 0; i < counts[m]; i++) tiles.push_back(TileRef{sids[m], i});
+        return;
+    }
+    int most = 0;
+    for (int c : counts) most = std::max(most, c);
+    for (int i = 0; i < most; i++)
+        for (size_t m = 0; m < counts.size(); m++)
+            if (i < counts[m]) tiles.push_back(TileRef{sids[m], i});
 }
 
 // descriptors and tile table of the attached members
@@ -61,6 +92,7 @@ int rebuild(sw2d_batch *b)
     desc.resize(top);
     memset(desc.data(), 0, sizeof(Scenario) * desc.size());
     int features = 0;
+    std::vector<int> counts, sids;
     for (sw2d_handle *h : b->members) {
         Scenario d = make_scenario(h);
         features |= features_of(d.s);
@@ -69,9 +101,11 @@ int rebuild(sw2d_batch *b)
         d.t_end = h->t_end;
         d.status = b->dstatus + h->member_slot;
         desc[h->member_slot] = d;
-        for (int i = 0; i < d.nblocks; i++) tiles.push_back(TileRef{h->member_slot, i});
+        counts.push_back(d.nblocks);
+        sids.push_back(h->member_slot);
         h->desc_dirty = true;  // its own descriptor is refreshed when it steps alone again
     }
+    order_tiles(counts, sids, b->order, tiles);
     if ((int)tiles.size() > b->tiles_cap) {
         bdrop_graph(b);
         if (b->dtiles) (void)hipFree(b->dtiles);
@@ -449,9 +483,10 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     // and write only the spare buffer and scratch bookkeeping.  trial(H, n):
     // mean time of n launches after one warm-up (negative when no member runs)
     const int phase = (int)(b->phase % 6);
-    auto trial = [&](int H, int n) -> float {
+    auto trial = [&](int H, int n, int order, int cap) -> float {
         std::vector<Scenario> desc(nm);
         std::vector<TileRef> tiles;
+        std::vector<int> counts, sids;
         int feats = 0;
         for (int i = 0; i < nm && e == hipSuccess; i++) {
             sw2d_handle *h = b->members[i];
@@ -470,10 +505,14 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
             d.phase0 = h->phase0;
             feats |= features_of(d.s);
             desc[i] = d;
-            for (int k = 0; k < d.nblocks; k++) tiles.push_back(TileRef{i, k});
+            counts.push_back(d.nblocks);
+            sids.push_back(i);
             e = hipMemcpyAsync(sc[i].clk, h->clk, 3 * sizeof(Clock), hipMemcpyDeviceToDevice, b->stream);
         }
+        order_tiles(counts, sids, order, tiles);
         if (e != hipSuccess || tiles.empty()) return -1.0f;
+        const size_t lds_live = b->lds_pad;
+        b->lds_pad = lds_for_cap(b->device, cap);
         e = hipMemcpy(dscen_s, desc.data(), sizeof(Scenario) * nm, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(dtiles_s, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess) e = blaunch(b, dscen_s, dtiles_s, (int)tiles.size(), phase, feats);
@@ -483,26 +522,49 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
         if (e == hipSuccess) e = hipEventSynchronize(e1);
         float ms = 0;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        b->lds_pad = lds_live;
+        if (e == hipSuccess) b->tune_log.push_back({(double)H, (double)order, (double)cap, ms / n});
         return e == hipSuccess ? ms / n : -1.0f;
     };
+    b->tune_log.clear();
     // two stages: a short trial of every candidate, then `iters` launches of
     // the three fastest
     int best = b->H;
     std::vector<std::pair<float, int>> t1;
     for (int H : cand) {
-        const float ms = trial(H, cand.size() > 3 ? std::max(1, iters / 4) : iters);
+        const float ms = trial(H, cand.size() > 3 ? std::max(1, iters / 4) : iters, b->order,
+                               b->waves_cap);
         if (ms < 0) break;
         t1.emplace_back(ms, H);
     }
     std::sort(t1.begin(), t1.end());
+    int best_order = b->order, best_cap = b->waves_cap;
     if (e == hipSuccess && !t1.empty()) {
         best = t1.front().second;
+        float best_ms = t1.front().first;
         if (t1.size() > 3) {
-            float best_ms = 1e30f;
+            best_ms = 1e30f;
             for (size_t i = 0; i < 3 && e == hipSuccess; i++) {
-                const float ms = trial(t1[i].second, iters);
+                const float ms = trial(t1[i].second, iters, b->order, b->waves_cap);
                 if (ms >= 0 && ms < best_ms) { best_ms = ms; best = t1[i].second; }
             }
+        }
+        // the launch schedule at that height: block order x a cap on the
+        // resident waves per SIMD below what the registers allow; a change is
+        // kept only when it is at least 1 % faster
+        const int natural = std::max(1, fused_waves_per_cu(b->precision, feats_all) / 4);
+        std::vector<std::pair<int, int>> sched = {{0, 0}, {1, 0}};
+        for (int cap = natural - 1; cap >= std::max(1, natural - 2); cap--) {
+            sched.push_back({0, cap});
+            sched.push_back({1, cap});
+        }
+        const float ref = trial(best, iters, b->order, b->waves_cap);
+        float pick_ms = ref;
+        for (auto [order, cap] : sched) {
+            if (e != hipSuccess) break;
+            if (order == b->order && cap == b->waves_cap) continue;
+            const float ms = trial(best, iters, order, cap);
+            if (ms >= 0 && ms < 0.99f * pick_ms) { pick_ms = ms; best_order = order; best_cap = cap; }
         }
     }
     (void)hipEventDestroy(e0);
@@ -510,6 +572,12 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     cleanup();
     if (e != hipSuccess) return bhip(b, e, "batch autotune");
     b->H = best;
+    if (best_order != b->order || best_cap != b->waves_cap) {
+        b->order = best_order;
+        b->waves_cap = best_cap;
+        b->lds_pad = lds_for_cap(b->device, best_cap);
+        bdrop_graph(b);
+    }
     // retiling folds each member's pending partial sums into one record
     for (sw2d_handle *h : b->members) {
         if (h->prm.tile_rows > 0) continue;
@@ -518,6 +586,38 @@ int sw2d_batch_autotune(sw2d_batch *b, int iters, int *tile_rows)
     b->dirty = true;
     BCHK(b, hipStreamSynchronize(b->stream));
     if (tile_rows) *tile_rows = best;
+    return SW2D_OK;
+}
+
+int sw2d_batch_set_schedule(sw2d_batch *b, int order, int waves_per_simd)
+{
+    if (!b) return SW2D_E_ARG;
+    if (order < 0 || order > 1 || waves_per_simd < 0 || waves_per_simd > 8)
+        return bfail(b, SW2D_E_ARG, "order must be 0 or 1 and waves_per_simd in [0, 8]");
+    BCHK(b, hipStreamSynchronize(b->stream));
+    b->order = order;
+    b->waves_cap = waves_per_simd;
+    b->lds_pad = lds_for_cap(b->device, waves_per_simd);
+    bdrop_graph(b);
+    b->dirty = true;
+    return SW2D_OK;
+}
+
+int sw2d_batch_schedule(sw2d_batch *b, int *order, int *waves_per_simd)
+{
+    if (!b) return SW2D_E_ARG;
+    if (order) *order = b->order;
+    if (waves_per_simd) *waves_per_simd = b->waves_cap;
+    return SW2D_OK;
+}
+
+int sw2d_batch_tune_report(sw2d_batch *b, double *rows, int cap, int *n)
+{
+    if (!b) return SW2D_E_ARG;
+    const int m = (int)b->tune_log.size();
+    for (int i = 0; i < std::min(m, cap) && rows; i++)
+        for (int k = 0; k < 4; k++) rows[4 * i + k] = b->tune_log[i][k];
+    if (n) *n = m;
     return SW2D_OK;
 }
 

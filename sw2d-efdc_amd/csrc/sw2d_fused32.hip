@@ -9,9 +9,9 @@
 namespace sw2d {
 
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase, int features)
+                           int phase, int features, size_t lds)
 {
-    return launch_fused_impl<float, float>(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_impl<float, float>(stream, scen, tiles, nblocks, phase, features, lds);
 }
 int fused_waves_per_cu_f32(int features) { return fused_waves_per_cu_impl<float, float>(features); }
 hipError_t launch_init_f32(sw2d_handle *h) { return launch_init_impl<float, float>(h); }

@@ -11,9 +11,9 @@ __global__ void __launch_bounds__(64) peek_kernel(const StepCommon s, Ledger *ou
 }
 
 hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase, int features)
+                           int phase, int features, size_t lds)
 {
-    return launch_fused_impl<double, double>(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_impl<double, double>(stream, scen, tiles, nblocks, phase, features, lds);
 }
 int fused_waves_per_cu_f64(int features) { return fused_waves_per_cu_impl<double, double>(features); }
 hipError_t launch_init_f64(sw2d_handle *h) { return launch_init_impl<double, double>(h); }

@@ -648,25 +648,27 @@ __global__ void __launch_bounds__(256) derive_field(const TH *h, const T *um, co
     }
 }
 
+// lds: dynamic LDS bytes reserved per block and left unused (0: none); a
+// batch uses it to cap the resident waves per SIMD
 template <typename T, typename TH>
 static hipError_t launch_fused_impl(hipStream_t stream, const Scenario *scen, const TileRef *tiles,
-                                    int nblocks, int phase, int features)
+                                    int nblocks, int phase, int features, size_t lds)
 {
     switch (fused_variant(features)) {
     case kVariant0:
-        hipLaunchKernelGGL((fused_step<T, TH, kVariant0>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant0>), dim3(nblocks), dim3(64), lds, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant1:
-        hipLaunchKernelGGL((fused_step<T, TH, kVariant1>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant1>), dim3(nblocks), dim3(64), lds, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant2:
-        hipLaunchKernelGGL((fused_step<T, TH, kVariant2>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant2>), dim3(nblocks), dim3(64), lds, stream, scen, tiles, nblocks, phase);
         break;
     case kVariant3:
-        hipLaunchKernelGGL((fused_step<T, TH, kVariant3>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kVariant3>), dim3(nblocks), dim3(64), lds, stream, scen, tiles, nblocks, phase);
         break;
     default:
-        hipLaunchKernelGGL((fused_step<T, TH, kFeatAll>), dim3(nblocks), dim3(64), 0, stream, scen, tiles, nblocks, phase);
+        hipLaunchKernelGGL((fused_step<T, TH, kFeatAll>), dim3(nblocks), dim3(64), lds, stream, scen, tiles, nblocks, phase);
         break;
     }
     return hipGetLastError();

@@ -10,9 +10,9 @@
 namespace sw2d {
 
 hipError_t launch_fused_mix(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                            int phase, int features)
+                            int phase, int features, size_t lds)
 {
-    return launch_fused_impl<float, double>(stream, scen, tiles, nblocks, phase, features);
+    return launch_fused_impl<float, double>(stream, scen, tiles, nblocks, phase, features, lds);
 }
 int fused_waves_per_cu_mix(int features) { return fused_waves_per_cu_impl<float, double>(features); }
 hipError_t launch_init_mix(sw2d_handle *h) { return launch_init_impl<float, double>(h); }

@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <array>
 #include <string>
 #include <vector>
 
@@ -247,6 +248,15 @@ struct sw2d_batch {
     char *pool_base = nullptr;
     sw2d::Pool pool;
     int H = 0;                              // member tile rows (0: each member's default)
+    // launch schedule (sw2d_batch_set_schedule): block order 0 = each
+    // member's tiles in a run, 1 = members interleaved tile by tile; a cap
+    // on resident waves per SIMD (0 = none), enforced by reserving lds_pad
+    // bytes of unused LDS per one-wave block
+    int order = 0;
+    int waves_cap = 0;
+    size_t lds_pad = 0;
+    // trials of the last sw2d_batch_autotune: (tile rows, order, waves cap, ms)
+    std::vector<std::array<double, 4>> tune_log;
     std::vector<sw2d_handle *> members;     // attached, in attach order
     std::vector<sw2d_handle *> owned;       // every handle made by sw2d_batch_new and not destroyed
     std::vector<int> free_slots;            // status / descriptor slots
@@ -274,12 +284,14 @@ constexpr int kMaxMembers = 4096;
 // scenario scen[0], blocks in order) at launch phase `phase`
 // `features` (kFeat*) covers every scenario of the launch; the smallest
 // instantiation of the kernel that handles them runs
+// lds: dynamic LDS bytes per block reserved and left unused (a cap on the
+// resident blocks per CU; 0 = none)
 hipError_t launch_fused_f64(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase, int features);
+                           int phase, int features, size_t lds = 0);
 hipError_t launch_fused_f32(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                           int phase, int features);
+                           int phase, int features, size_t lds = 0);
 hipError_t launch_fused_mix(hipStream_t stream, const Scenario *scen, const TileRef *tiles, int nblocks,
-                            int phase, int features);
+                            int phase, int features, size_t lds = 0);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
 hipError_t launch_init_mix(sw2d_handle *h);
@@ -302,11 +314,12 @@ inline bool known_precision(int prec)
 }
 // the fused kernel of a precision form
 inline hipError_t launch_fused(int prec, hipStream_t stream, const Scenario *scen,
-                               const TileRef *tiles, int nblocks, int phase, int features)
+                               const TileRef *tiles, int nblocks, int phase, int features,
+                               size_t lds = 0)
 {
-    if (prec == SW2D_FP64) return launch_fused_f64(stream, scen, tiles, nblocks, phase, features);
-    if (prec == SW2D_FP32) return launch_fused_f32(stream, scen, tiles, nblocks, phase, features);
-    return launch_fused_mix(stream, scen, tiles, nblocks, phase, features);
+    if (prec == SW2D_FP64) return launch_fused_f64(stream, scen, tiles, nblocks, phase, features, lds);
+    if (prec == SW2D_FP32) return launch_fused_f32(stream, scen, tiles, nblocks, phase, features, lds);
+    return launch_fused_mix(stream, scen, tiles, nblocks, phase, features, lds);
 }
 inline int fused_waves_per_cu(int prec, int features)
 {

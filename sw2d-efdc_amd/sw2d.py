@@ -123,6 +123,9 @@ def lib():
         "sw2d_time": (ct.c_int, [_H, _DP, ct.POINTER(ct.c_long), _DP]),
         "sw2d_get_field": (ct.c_int, [_H, ct.c_int, _DP]),
         "sw2d_get_field_noalloc": (ct.c_int, [_H, ct.c_int, _DP]),
+        "sw2d_batch_set_schedule": (ct.c_int, [_H, ct.c_int, ct.c_int]),
+        "sw2d_batch_schedule": (ct.c_int, [_H, _IP, _IP]),
+        "sw2d_batch_tune_report": (ct.c_int, [_H, _DP, ct.c_int, _IP]),
         "sw2d_snapshot": (ct.c_int, [_H, ct.c_uint, _D, ct.POINTER(ct.c_long)]),
         "sw2d_snapshot_field": (ct.c_int, [_H, ct.c_long, ct.c_int, ct.POINTER(_DP)]),
         "sw2d_snapshot_release": (ct.c_int, [_H, ct.c_long]),
@@ -545,6 +548,25 @@ class Batch:
         best = ct.c_int(0)
         self._chk(self._L.sw2d_batch_autotune(self._b, iters, ct.byref(best)), "batch autotune")
         return best.value
+
+    def set_schedule(self, order=0, waves_per_simd=0):
+        """sw2d_batch_set_schedule: block order (0 runs, 1 interleaved) and a
+        cap on resident waves per SIMD (0 = none)."""
+        self._chk(self._L.sw2d_batch_set_schedule(self._b, order, waves_per_simd), "set_schedule")
+
+    def schedule(self):
+        o, w = ct.c_int(0), ct.c_int(0)
+        self._chk(self._L.sw2d_batch_schedule(self._b, ct.byref(o), ct.byref(w)), "schedule")
+        return o.value, w.value
+
+    def tune_report(self):
+        """Trials of the last autotune: rows (tile rows, order, waves cap, ms)."""
+        n = ct.c_int(0)
+        self._chk(self._L.sw2d_batch_tune_report(self._b, None, 0, ct.byref(n)), "tune_report")
+        out = np.zeros(4 * max(n.value, 1))
+        self._chk(self._L.sw2d_batch_tune_report(self._b, _dp(out), n.value, ct.byref(n)),
+                  "tune_report")
+        return out[: 4 * n.value].reshape(n.value, 4)
 
     def memory(self):
         a, b, c = ct.c_size_t(0), ct.c_size_t(0), ct.c_size_t(0)

@@ -252,3 +252,67 @@ def test_batch_autotune_with_a_member_of_fixed_small_tiles(pkg, cases):
     for k, m in enumerate(members):
         assert_same(m, alone[k], f"member {k}")
     batch.close()
+
+
+@pytest.mark.parametrize("order,waves", [(1, 0), (0, 2), (1, 1)])
+def test_batch_schedule_keeps_results(pkg, cases, order, waves):
+    """The launch schedule (block order, a cap on resident waves per SIMD)
+    changes when and where tiles run, never the numbers: members equal
+    their stand-alone runs value for value, set before the run and changed
+    mid-run (graph replays captured afresh)."""
+    batch = pkg.Batch(precision="fp32", pool_bytes=256 << 20, tile_rows=8)
+    batch.set_schedule(order, waves)
+    assert batch.schedule() == (order, waves)
+    alone, members = [], []
+    for i in range(4):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision="fp32", tile_rows=8, **kw)
+        setup(a, cases, i, rain, outlet)
+        a.step(40)
+        alone.append(a)
+        m = batch.case(z, h0, 10.0, **kw)
+        setup(m, cases, i, rain, outlet)
+        batch.attach(m, max_iters=40)
+        members.append(m)
+    batch.step(13)
+    batch.set_schedule(1 - order, 0)
+    while batch.step(6):
+        pass
+    for i in range(4):
+        assert_same(members[i], alone[i], f"schedule ({order}, {waves}) member {i}")
+        assert np.array_equal(members[i].hydrograph(), alone[i].hydrograph())
+    batch.close()
+
+
+def test_batch_autotune_searches_the_schedule(pkg, cases):
+    """sw2d_batch_autotune times tile heights, then both block orders and
+    caps below the register-limited occupancy at the chosen height; its
+    report lists every trial and the pick is one of them; results are those
+    of the stand-alone runs."""
+    batch = pkg.Batch(precision="fp32", pool_bytes=256 << 20)
+    alone, members = [], []
+    for i in range(5):
+        z, h0, kw, rain, outlet = scenario(cases, i)
+        a = pkg.Case(z, h0, 10.0, precision="fp32", **kw)
+        setup(a, cases, i, rain, outlet)
+        alone.append(a)
+        m = batch.case(z, h0, 10.0, **kw)
+        setup(m, cases, i, rain, outlet)
+        batch.attach(m, max_iters=36)
+        members.append(m)
+    batch.step(12)
+    H = batch.autotune(2)
+    rep = batch.tune_report()
+    assert rep.shape[1] == 4 and len(rep) >= 4
+    assert set(rep[:, 1].astype(int)) == {0, 1}                 # both orders tried
+    assert (rep[:, 2] > 0).any()                                # some cap tried
+    assert np.all(rep[:, 3] > 0)
+    order, waves = batch.schedule()
+    assert any(r[0] == H and r[1] == order and r[2] == waves for r in rep)
+    while batch.step(6):
+        pass
+    for i in range(5):
+        alone[i].step(36)
+        # the state does not depend on the tiling or the schedule
+        assert_same(members[i], alone[i], f"tuned member {i}")
+    batch.close()
