@@ -430,15 +430,34 @@ def _esri(path, a, cellsize, nodata, fmt, xll=0.0, yll=0.0):
             f.write(" ".join(fmt(v) for v in row) + "\n")
 
 
+def efdc_card_template():
+    """A small EFDC.INP card template (CRLF lines, as EFDC's own): cards C1 to
+    C91 with C11A and C71A, each a title line, a comment, a header line, one
+    data line and a dashed line."""
+    names = []
+    for k in range(1, 92):
+        names.append(f"C{k}")
+        if k in (11, 71):
+            names.append(f"C{k}A")
+    out = ["C ** EFDC.INP card template  ", "C **  "]
+    for i, nm in enumerate(names):
+        out += [f"{nm} TITLE OF CARD {nm}  ", "*  ", f"*  {nm}X: A VALUE OF CARD {nm}  ",
+                f"{nm}  {nm}X  {nm}Y  ", f"    {i}    {i + 1}  ", "-" * 79 + "  "]
+    return ("\r\n".join(out) + "\r\n").encode()
+
+
 def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=None,
              efdc=False, basetime=86400.0, nlayers=1, evaporation=None, INT=0.0, INF=0.0,
-             LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1), efdc_values=None, met=None):
+             LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1), efdc_values=None, met=None,
+             efdc_points=((15.0, 25.0),), tctmsr=1.0, efdc_template=None):
     """Write db/dir.txt and db/<name>/ for a fixed-step spec in the reference's
     layout.  probes: (x, y) metres for coord_out.dat; sources: (xs, ys, Q, dtoq);
     evaporation: (albedo, T[], Rg[]) with one value per rain interval.
     efdc_values: EFDC block values by EFDC_NAMES name; met: (patm[], relh[],
     wind[(speed, direction)]) for atm_press.dat, rel_humidity.dat and
-    wind_velocities.dat (constant series when None)."""
+    wind_velocities.dat (constant series when None); efdc_points, tctmsr:
+    efdc_coord_out.dat; efdc_template: bytes of EFDC/input/template/EFDC.INP
+    (efdc_card_template() when None)."""
     import os
     base = os.path.join(root, "db", name)
     for d in ("SW2D/input", "SW2D/output", "EFDC/input", "EFDC/output"):
@@ -500,4 +519,11 @@ def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=N
             f.write("rel_humidity(percent)\n" + "".join(f"{float(v)!r}\n" for v in relh))
         with open(os.path.join(inp, "wind_velocities.dat"), "w") as f:
             f.write("".join(f"{float(a)!r}\t{float(b)!r}\n" for a, b in wind))
+        with open(os.path.join(inp, "efdc_coord_out.dat"), "w") as f:
+            f.write(f"time_conversion: {float(tctmsr)!r}\nnumber_t_s_out: {len(efdc_points)}\n" +
+                    "".join(f"{float(x)!r} {float(y)!r}\n" for x, y in efdc_points))
+        tdir = os.path.join(base, "EFDC", "input", "template")
+        os.makedirs(tdir, exist_ok=True)
+        with open(os.path.join(tdir, "EFDC.INP"), "wb") as f:
+            f.write(efdc_template if efdc_template is not None else efdc_card_template())
     return base

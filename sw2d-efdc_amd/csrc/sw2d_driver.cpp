@@ -9,13 +9,16 @@
 // EFDC/input/QSER.INP (couple_BC_efdc + write_efdc_QSER, with the
 // reference's one-output lag of the velocities).  With the coupling on it
 // also writes EFDC's series and initial-condition inputs (WSER, ASER, SHOW,
-// DYE/SALT/TEMP.INP, DSER/TSER/SSER).  EFDC.INP card patching, GEFDC/Gorp and
-// the mpirun of the Fortran lake model are not part of this build.
+// DYE/SALT/TEMP.INP, DSER/TSER/SSER) and EFDC/input/EFDC.INP: a copy of
+// EFDC/input/template/EFDC.INP made before the run (4281-4289) whose cards
+// C6-C11, C14, C22-C26, C71-C73, C84 and C87 are rewritten with the first
+// output (4924-4933).  GEFDC/Gorp and the mpirun of the Fortran lake model
+// are not part of this build.
 //
 // usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32|mixed]
 //                  [--device N] [--autotune] [--parse-only] [--efdc-inputs]
 // --efdc-inputs writes the EFDC inputs known before the run (WSER, ASER,
-// SHOW, DYE/SALT/TEMP.INP) and stops; no GPU is used.
+// SHOW, DYE/SALT/TEMP.INP, EFDC.INP) and stops; no GPU is used.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -162,6 +165,105 @@ void print_parse_summary(const sw2dio::Case &c, double time0, double timmax)
 // 4588-4612 initial constituent fields, 4767-4785 atmospheric series).
 // Series shorter than the run's EFDC record count end the run with a
 // message (the reference reads past them).
+// byte copy of the card template (4281-4289)
+bool copy_file(const std::string &from, const std::string &to)
+{
+    FILE *in = fopen(from.c_str(), "rb");
+    if (!in) {
+        printf("unknown file - %s\n", from.c_str());
+        return false;
+    }
+    FILE *out = fopen(to.c_str(), "wb");
+    bool ok = out != nullptr;
+    char buf[65536];
+    size_t n;
+    while (ok && (n = fread(buf, 1, sizeof buf, in)) > 0) ok = fwrite(buf, 1, n, out) == n;
+    ok = ok && !ferror(in);
+    fclose(in);
+    if (out && fclose(out) != 0) ok = false;
+    if (!ok) printf(" cannot write %s\n", to.c_str());
+    return ok;
+}
+
+// The EFDC.INP card values known before the run (4288-4334, 4296-4307 and
+// setup.dat's EFDC block); the coupling records are added with the first
+// output.  water = lake cells of mask.dat.
+sw2dio::EfdcConfig efdc_config(const sw2dio::Case &c, double timmax, int water)
+{
+    const sw2dio::Setup &s = c.setup;
+    const std::vector<double> &b = s.efdc_block;
+    auto I = [&](int k) { return (int)b[k]; };
+    sw2dio::EfdcConfig e;
+    e.nlayers = s.nlayers;
+    e.ncols = c.dem.cols;
+    e.nrows = c.dem.rows;
+    e.nwcells = water;
+    e.TCON = s.efdc_basetime;
+    e.TREF = s.efdc_basetime;
+    e.TBEGIN = s.TBEGIN;
+    e.NTC = (int)(timmax / s.efdc_basetime);
+    e.NTSPTC = s.NTSPTC;
+    e.NTCPP = e.NTC;
+    e.NTCVB = e.NTC;
+    e.ISPPH = I(sw2dio::kISPPH); e.NPPPH = I(sw2dio::kNPPPH);
+    e.ISRPPH = I(sw2dio::kISRPPH); e.IPPHXY = I(sw2dio::kIPPHXY);
+    e.ISVPH = I(sw2dio::kISVPH); e.NPVPH = I(sw2dio::kNPVPH);
+    e.ISRVPH = I(sw2dio::kISRVPH); e.IVPHXY = I(sw2dio::kIVPHXY);
+    e.ISTMSR = s.out_outlet;
+    e.MLTMSR = (int)c.efdc_out_x.size();
+    e.NBTMSR = 1;
+    e.NSTMSR = e.NTC * e.NTSPTC;
+    e.NWTMSR = e.NTSPTC;
+    e.NTSSTSP = 1;
+    e.TCTMSR = c.efdc_tctmsr;
+    // (I, J) of a location: cells from the DEM's lower-left corner (4331-4332)
+    const double res = c.dem.cellsize;
+    for (size_t i = 0; i < c.efdc_out_x.size(); i++)
+        e.LTS.push_back({(int)round(fabs((c.dem.xll - c.efdc_out_x[i]) / res)),
+                         (int)round(fabs((c.dem.yll - c.efdc_out_y[i]) / res))});
+    e.NTSSSS = I(sw2dio::kNTSSSS); e.MTSP = I(sw2dio::kMTSP); e.MTSC = I(sw2dio::kMTSC);
+    e.MTSA = I(sw2dio::kMTSA); e.MTSUE = I(sw2dio::kMTSUE); e.MTSUT = I(sw2dio::kMTSUT);
+    e.MTSU = I(sw2dio::kMTSU);
+    e.ZBRADJ = b[sw2dio::kZBRADJ]; e.ZBRCVRT = b[sw2dio::kZBRCVRT]; e.HMIN = b[sw2dio::kHMIN];
+    e.HADJ = b[sw2dio::kHADJ]; e.HCVRT = b[sw2dio::kHCVRT]; e.HDRY = b[sw2dio::kHDRY];
+    e.HWET = b[sw2dio::kHWET]; e.BELADJ = b[sw2dio::kBELADJ]; e.BELCVRT = b[sw2dio::kBELCVRT];
+    e.NWSER = I(sw2dio::kNWSER);
+    e.NASER = I(sw2dio::kNASER);
+    e.DX = res;
+    e.DY = res;
+    e.dye_on = I(sw2dio::kDyeOn);
+    e.salt_on = I(sw2dio::kSaltOn);
+    e.temp_on = I(sw2dio::kTempOn);
+    e.temp_opt = 1;
+    return e;
+}
+
+// records come back from sw2d_couple_bc as S, N, E, W; QSER.INP and C24
+// list them N, S, E, W
+const int kind_of_edge[4] = {1, 0, 2, 3};
+
+// EFDC/input/EFDC.INP's cards with the coupling records of the first output
+// (4924-4933); I/J/counts as sw2d_couple_bc returns them, lists cap apart
+bool write_efdc_cards(const sw2dio::Case &c, sw2dio::EfdcConfig cfg, const int *I, const int *J,
+                      const int *counts, int cap)
+{
+    int n = 0;
+    for (int e = 0; e < 4; e++) {
+        const int k = kind_of_edge[e];
+        cfg.bc[e].clear();
+        for (int i = 0; i < counts[k]; i++)
+            cfg.bc[e].push_back({I[(size_t)k * cap + i], J[(size_t)k * cap + i]});
+        n += counts[k];
+    }
+    cfg.nqsij = cfg.nqser = n;
+    std::string err;
+    if (!sw2dio::write_efdc_config(c.dir + "/EFDC/input/EFDC.INP", cfg, &err)) {
+        printf(" EFDC.INP: %s\n", err.c_str());
+        return false;
+    }
+    return true;
+}
+
 bool write_efdc_setup_files(const sw2dio::Case &c, const std::vector<double> &ev, double time0,
                             double timmax, long lkout)
 {
@@ -270,6 +372,25 @@ int run_case(const Options &opt, const std::string &name)
             printf(" cannot write %s/EFDC/input/SHOW.INP\n", c.dir.c_str());
             return 1;
         }
+        // EFDC.INP: the first output's coupling records depend only on
+        // mask.dat (couple_BC_efdc at output 0 takes the initial depths and
+        // zero velocities, 4902), so its cards are known before the run
+        if (!copy_file(c.dir + "/EFDC/input/template/EFDC.INP", c.dir + "/EFDC/input/EFDC.INP"))
+            return 1;
+        const size_t RC = (size_t)R * C;
+        const int cap = (int)RC;
+        std::vector<double> zero(RC, 0.0), q(4 * RC);
+        std::vector<int> cells(4 * RC), I(4 * RC), J(4 * RC), counts(4);
+        if (sw2d_couple_bc(R, C, c.dem.cellsize, c.dem.cellsize, c.mask.data(), c.h0.data(), zero.data(),
+                           zero.data(), cap, cells.data(), I.data(), J.data(), q.data(),
+                           counts.data()) != SW2D_OK) {
+            printf(" sw2d_couple_bc failed: %s\n", sw2d_last_error(nullptr));
+            return 1;
+        }
+        int water = 0;
+        for (int m : c.mask) water += (m == 1);
+        if (!write_efdc_cards(c, efdc_config(c, timmax, water), I.data(), J.data(), counts.data(), cap))
+            return 1;
         return 0;
     }
 
@@ -310,6 +431,8 @@ int run_case(const Options &opt, const std::string &name)
         ok = check(sw2d_set_evaporation(h, ev.data(), (int)ev.size()), "evaporation");
     }
     if (ok && s.efdc == 1) ok = write_efdc_setup_files(c, ev, time0, timmax, lkout);
+    if (ok && s.efdc == 1)
+        ok = copy_file(c.dir + "/EFDC/input/template/EFDC.INP", c.dir + "/EFDC/input/EFDC.INP");
     if (ok && opt.autotune) {
         int H = 0;
         ok = check(sw2d_autotune(h, 10, &H), "autotune");
@@ -344,7 +467,12 @@ int run_case(const Options &opt, const std::string &name)
     if (s.efdc == 1) {
         cells.resize(4 * RC); I.resize(4 * RC); J.resize(4 * RC); q.resize(4 * RC);
     }
-    bool efdc_note = false;
+    sw2dio::EfdcConfig efdc_cards;
+    if (s.efdc == 1) {
+        int water = 0;
+        for (int m : c.mask) water += (m == 1);
+        efdc_cards = efdc_config(c, timmax, water);
+    }
 
     AsyncWriter writer(2);
     std::atomic<bool> write_failed{false};  // set by the writer thread
@@ -389,12 +517,10 @@ int run_case(const Options &opt, const std::string &name)
             run_failed = true;
             break;
         }
-        if (s.efdc == 1 && tq == 0 && !efdc_note) {
-            // written with the first output, beside the EFDC.INP card update (4924-4938)
+        if (s.efdc == 1 && tq == 0) {
+            // written with the first output, beside the EFDC.INP card update (4934-4936)
             const double *sv = &s.efdc_block[sw2dio::kNSTYPE];
             if (!sw2dio::write_show(c.dir + "/EFDC/input/SHOW.INP", sv)) write_failed = true;
-            printf(" EFDC.INP card updates are not written by this build\n");
-            efdc_note = true;
         }
         // the coupling state (records, QSER series, lagged velocities) and the
         // files belong to the writer thread; jobs run in output order
@@ -413,8 +539,6 @@ int run_case(const Options &opt, const std::string &name)
                 if (hh && ua && va) {
                     sw2d_couple_bc(R, C, res, res, c.mask.data(), hh, vva_host.data(), uua_host.data(), cap,
                                    cells.data(), I.data(), J.data(), q.data(), counts.data());
-                    // records come back S, N, E, W; QSER.INP lists N, S, E, W
-                    static const int kind_of_edge[4] = {1, 0, 2, 3};
                     for (int e = 0; e < 4; e++) {
                         const int k = kind_of_edge[e];
                         qser.nrec[e] = counts[k];
@@ -422,6 +546,8 @@ int run_case(const Options &opt, const std::string &name)
                         qser.val[e].emplace_back(q.begin() + (size_t)k * cap,
                                                  q.begin() + (size_t)k * cap + counts[k]);
                     }
+                    if (tq == 0 && !write_efdc_cards(c, efdc_cards, I.data(), J.data(), counts.data(), cap))
+                        write_failed = true;
                     // the velocities used by the next output are this
                     // output's, taken after the coupling call (4913-4916)
                     std::copy(ua, ua + RC, uua_host.begin());

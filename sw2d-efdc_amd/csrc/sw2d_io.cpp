@@ -348,6 +348,22 @@ std::string read_case(const std::string &root, const std::string &name, Case &c)
         }
     }
     if (c.setup.efdc == 1) {
+        // EFDC time-series locations (4310-4334)
+        File g(in + "efdc_coord_out.dat", "r");
+        if (!g) return "unknown file - efdc_coord_out.dat";
+        int n = 0;
+        if (fscanf(g.fp, "time_conversion:  %lf\n ", &c.efdc_tctmsr) != 1 ||
+            fscanf(g.fp, "number_t_s_out:  %d\n ", &n) != 1 || n < 0)
+            return "malformed efdc_coord_out.dat";
+        for (int i = 0; i < n; i++) {
+            double x = 0, y = 0;
+            if (fscanf(g.fp, " %lf %lf\n ", &x, &y) != 2)
+                return "efdc_coord_out.dat holds fewer than number_t_s_out points";
+            c.efdc_out_x.push_back(x);
+            c.efdc_out_y.push_back(y);
+        }
+    }
+    if (c.setup.efdc == 1) {
         File f(in + "mask.dat", "r");
         if (!f) return "unknown file - mask.dat";
         if (!read_esri_header(f.fp, c.mask_hdr)) return "malformed mask.dat header";

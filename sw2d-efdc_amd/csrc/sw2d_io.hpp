@@ -6,6 +6,7 @@
 #pragma once
 #include <stdio.h>
 
+#include <array>
 #include <functional>
 #include <string>
 #include <vector>
@@ -62,6 +63,9 @@ struct Case {
     Grid mask_hdr;                     // mask.dat header (efdc == 1)
     std::vector<double> patm, relh;    // atm_press.dat, rel_humidity.dat (efdc == 1)
     std::vector<double> wind;          // wind_velocities.dat pairs (efdc == 1, NWSER == 1)
+    // efdc_coord_out.dat (efdc == 1, 4310-4334): EFDC time-series locations
+    double efdc_tctmsr = 0;            // time_conversion (C84 TCTMSR)
+    std::vector<double> efdc_out_x, efdc_out_y;  // metres
 };
 
 // count of '\n' in a file (the reference sizes series this way, 4041-4048)
@@ -125,5 +129,32 @@ bool write_aser(const std::string &path, const std::vector<double> &rows7, const
 
 // SHOW.INP (write_efdc_SHOW 2745-2764): the eight values from kNSTYPE
 bool write_show(const std::string &path, const double *show);
+
+// The values of the EFDC.INP cards that the coupled run rewrites
+// (write_efdc_CONFIG, 1069-2024; arguments as passed at 4925-4933).
+struct EfdcConfig {
+    int nqsij = 0, nqser = 0;                 // coupling records (C23)
+    std::vector<std::array<int, 2>> bc[4];    // (I, J) of the records, edges N, S, E, W (C24)
+    int nlayers = 1, ncols = 0, nrows = 0, nwcells = 0;  // C9, C10
+    double TCON = 0, TBEGIN = 0, TREF = 0;    // C8
+    int NTC = 0, NTSPTC = 0, NTCPP = 0, NTCVB = 0;  // C7
+    int ISPPH = 0, NPPPH = 0, ISRPPH = 0, IPPHXY = 0;  // C72
+    int ISVPH = 0, NPVPH = 0, ISRVPH = 0, IVPHXY = 0;  // C73
+    int ISTMSR = 0, MLTMSR = 0, NBTMSR = 0, NSTMSR = 0, NWTMSR = 0, NTSSTSP = 0;  // C84
+    double TCTMSR = 0;
+    std::vector<std::array<int, 2>> LTS;      // (I, J) of the time-series locations (C87)
+    int NTSSSS = 0, MTSP = 0, MTSC = 0, MTSA = 0, MTSUE = 0, MTSUT = 0, MTSU = 0;
+    double ZBRADJ = 0, ZBRCVRT = 0, HMIN = 0, HADJ = 0, HCVRT = 0, HDRY = 0, HWET = 0,
+           BELADJ = 0, BELCVRT = 0;           // C11
+    int NWSER = 0, NASER = 0;                 // C14
+    double DX = 0, DY = 0;
+    int dye_on = 0, salt_on = 0, temp_on = 0, temp_opt = 0;  // C6, C22, C24, C71
+};
+
+// Rewrites cards C6-C11, C14, C22-C26, C71-C73, C84 and C87 of the EFDC.INP
+// at `path` in place of the template's; every byte outside those cards is
+// kept.  false (and *err) when the file cannot be read or written or lacks
+// one of the cards or their successors.
+bool write_efdc_config(const std::string &path, const EfdcConfig &c, std::string *err = nullptr);
 
 }  // namespace sw2dio

@@ -5,8 +5,15 @@ LabHidro/SW2D-EFDC (a checkout at /root/reference by default) and stores them
 as compressed arrays in tests/golden/peri_lake.npz.  These are input data
 files of the reference's only example case (556 x 701, dx = 10 m); its DEM is
 not part of the repository, so no flow results exist to compare against.
+
+The case's EFDC card template (EFDC/input/template/EFDC.INP, the file the
+coupled run copies and rewrites cards of) and its EFDC time-series locations
+(SW2D/input/efdc_coord_out.dat) are stored as they are, the template
+gzip-compressed: peri_lake_EFDC.INP.gz, peri_lake_efdc_coord_out.dat.
 """
+import gzip
 import pathlib
+import shutil
 import sys
 
 import numpy as np
@@ -35,6 +42,12 @@ def main(ref="/root/reference"):
                         header=np.array([hdr["ncols"], hdr["nrows"], hdr["xllcorner"],
                                          hdr["yllcorner"], hdr["cellsize"], hdr["nodata_value"]]))
     print(R, C, int((mask == 1).sum()), h0.max())
+    root = pathlib.Path(ref) / "db" / "COUPLED_peri_lake"
+    raw = (root / "EFDC" / "input" / "template" / "EFDC.INP").read_bytes()
+    with gzip.GzipFile(HERE / "peri_lake_EFDC.INP.gz", "wb", mtime=0) as g:
+        g.write(raw)
+    shutil.copyfile(case / "efdc_coord_out.dat", HERE / "peri_lake_efdc_coord_out.dat")
+    print(len(raw), "bytes of EFDC.INP")
 
 
 if __name__ == "__main__":

@@ -302,6 +302,22 @@ def test_driver_outputs_match_oracle(cases, po, tmp_path, kernel):
     assert (inp / "SSER.INP").read_text() == series_text("S", times, nrec, lambda t: ev["SALT_IN"], nl, bt)
     assert (inp / "TSER.INP").read_text() == series_text("T", times, nrec, lambda t: T[t], nl, bt)
     assert (inp / "SHOW.INP").is_file() and (inp / "TEMP.INP").is_file()
+    # EFDC.INP, rewritten with the first output: C23's record counts and
+    # C24's records per edge are QSER.INP's series
+    from test_efdc_config import EfdcReader
+    efdc = (inp / "EFDC.INP").read_bytes()
+    rd = EfdcReader(efdc)
+    rd.seek("C23")
+    c23 = [int(v) for v in rd.read(12)]
+    assert c23[4] == c23[6] == sum(nrec) > 0
+    qser = (inp / "QSER.INP").read_text()
+    for e, n in zip(("North", "South", "East", "West"), nrec):
+        assert qser.count(f"' *** {e}_edge_") == n
+        assert efdc.decode("latin-1").count(f"! {e}_bc") == n
+    rd.seek("C24")
+    ij = [tuple(int(v) for v in rd.read(14)[:2]) for _ in range(sum(nrec))]
+    assert ij == [(int(i), int(j)) for e in ("north", "south", "east", "west")
+                  for i, j in zip(hist[0][e]["I"], hist[0][e]["J"])]
 
 
 @pytest.mark.gpu
