@@ -14,7 +14,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SW2D_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of `make -C oracle asan`)
+LIB_PATH = os.environ.get("SW2D_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 F_H, F_UM, F_VN, F_UU, F_VV, F_UUA, F_VVA, F_Z, F_HM, F_HN, F_UU1, F_VV1, F_INF = range(13)
 FIELDS = {"h": F_H, "um": F_UM, "vn": F_VN, "uu": F_UU, "vv": F_VV, "uua": F_UUA,

@@ -16,7 +16,8 @@ import numpy as np
 import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
-DRIVER = ROOT / "sw2d-efdc_amd" / "sw2d-efdc"
+# SW2D_DRIVER selects another build of the driver (the sanitizer build)
+DRIVER = pathlib.Path(os.environ.get("SW2D_DRIVER") or ROOT / "sw2d-efdc_amd" / "sw2d-efdc")
 
 
 def small_case(cases, R=24, C=37, t_end=480.0):

@@ -21,6 +21,7 @@ in-place rewrite leaves the end of the file damaged (it drops or adds a byte
 per card), so no output of it is a usable golden file.
 """
 import gzip
+import os
 import pathlib
 import re
 import subprocess
@@ -29,7 +30,8 @@ import numpy as np
 import pytest
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
-DRIVER = ROOT / "sw2d-efdc_amd" / "sw2d-efdc"
+# SW2D_DRIVER selects another build of the driver (the sanitizer build)
+DRIVER = pathlib.Path(os.environ.get("SW2D_DRIVER") or ROOT / "sw2d-efdc_amd" / "sw2d-efdc")
 GOLDEN = ROOT / "tests" / "golden"
 
 # rewritten cards and the card after each (the end of its block)
