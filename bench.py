@@ -710,6 +710,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-config0", dest="config0", action="store_false",
                     help="skip the whole 256x256 run (GPU vs CPU oracle)")
+    ap.add_argument("--legs", default="serve,sustained,batch,dropin,output,pond,config0,cpu",
+                    help="comma list of the legs to run (default: all); without serve the line "
+                         "carries no headline value (profiling runs of single legs)")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="after the run, stream this many MB through the library's probe kernel "
                          "(reads and writes n doubles: a copy of known size, to scale rocprofv3 "
@@ -729,16 +732,24 @@ def main():
     copy_gbs = pkg.sw2d.stream_copy_gbs(1 << 30, 20)
     log(f"stream copy ceiling: {copy_gbs:.0f} GB/s")
 
+    legs = set(args.legs.split(","))
+
     # ---- headline: the serving session ----
-    sv = serve_leg(pkg, cases, args.serve_seconds, args.serve_rate, args.serve_hours,
-                   args.serve_precision, args.serve_active_cells, args.serve_tile_rows,
-                   args.serve_err_sample, plain=args.serve_plain)
-    log(f"serving: {sv['value']:.3e} cell-updates/s, p50 {sv['p50_time_to_hydrograph_s']:.2f} s, "
-        f"p99 {sv['p99_time_to_hydrograph_s']:.2f} s")
+    if "serve" in legs:
+        sv = serve_leg(pkg, cases, args.serve_seconds, args.serve_rate, args.serve_hours,
+                       args.serve_precision, args.serve_active_cells, args.serve_tile_rows,
+                       args.serve_err_sample, plain=args.serve_plain)
+        log(f"serving: {sv['value']:.3e} cell-updates/s, p50 {sv['p50_time_to_hydrograph_s']:.2f} "
+            f"s, p99 {sv['p99_time_to_hydrograph_s']:.2f} s")
+    else:
+        sv = {"value": None, "wall_s": 0.0, "busy_s": 0.0, "launches": 0, "requests": 0,
+              "workload": "not run",
+              "roofline": {"achieved": None, "frac": None, "bytes_per_cell_update": None,
+                           "kernel_avg_us": None}}
     ms_per_step = 1e3 * sv["busy_s"] / max(sv["launches"], 1)
     rf = sv["roofline"]
     traffic_bpc = serve_traffic(args.serve_precision)
-    cells_per_launch = sv["value"] * sv["wall_s"] / max(sv["launches"], 1)
+    cells_per_launch = (sv["value"] or 0.0) * sv["wall_s"] / max(sv["launches"], 1)
     out = {
         "metric": METRIC,
         "value": sv["value"],
@@ -782,7 +793,7 @@ def main():
         },
         "serving": sv,
     }
-    if args.sustained_rate > 0:
+    if args.sustained_rate > 0 and "sustained" in legs:
         # the same request mix at a rate one GPU keeps up with: latency
         # without a standing queue
         out["serving_sustained"] = serve_leg(pkg, cases, args.sustained_seconds,
@@ -792,24 +803,24 @@ def main():
                                              label="sustained")
         log(f"sustained {args.sustained_rate}/s: p50 "
             f"{out['serving_sustained']['p50_time_to_hydrograph_s']:.2f} s")
-    if args.batch_scenarios > 0:
+    if args.batch_scenarios > 0 and "batch" in legs:
         out["batch"] = batch_leg(pkg, cases, args.batch_scenarios, args.batch_plain_sample)
-    if args.dropin:
+    if args.dropin and "dropin" in legs:
         out["dropin"] = dropin_legs(pkg, cases, args.size, args.steps, args.warmup,
                                     args.wet_time, args.plain_steps)
-        if args.output_every > 0:
-            spec = cases.config1(rows=args.size, cols=args.size)
-            out["output_path"] = output_leg(pkg, spec, args.output_iters, args.output_every)
-            oc = out["output_path"]["output_cost_ms"]
-            log(f"outputs: {oc['blocking']:.1f} ms each blocking, {oc['snapshot']:.1f} ms as snapshots")
-    if args.pond_break > 0:
+    if args.output_every > 0 and "output" in legs:
+        spec = cases.config1(rows=args.size, cols=args.size)
+        out["output_path"] = output_leg(pkg, spec, args.output_iters, args.output_every)
+        oc = out["output_path"]["output_cost_ms"]
+        log(f"outputs: {oc['blocking']:.1f} ms each blocking, {oc['snapshot']:.1f} ms as snapshots")
+    if args.pond_break > 0 and "pond" in legs:
         out["precision_study"] = pond_break_leg(pkg, cases, args.pond_break)
         log(f"pond break: fp32 max|dh| {out['precision_study']['fp32']['max_abs_dh_m']:.2e} m")
     c0_run = None
-    if not args.no_cpu and args.config0:
+    if not args.no_cpu and args.config0 and "config0" in legs:
         out["config0"], c0_run = config0_leg(pkg, cases)
         log(f"reference 256^2 case: GPU {out['config0']['gpu_fp64_s']:.3f} s")
-    if not args.no_cpu:
+    if not args.no_cpu and "cpu" in legs:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline: {threads} threads, {args.cpu_seconds} s")
         out["cpu_baseline"], c0_cpu = cpu_baseline(cases, args.cpu_seconds, threads,
