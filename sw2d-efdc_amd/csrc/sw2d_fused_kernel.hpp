@@ -164,6 +164,9 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const T dt2 = (T)st.dt2;
     const TH dt2h = (TH)st.dt2;
     const T dt2dx = (T)(st.dt2 / s.dx), dt2dy = (T)(st.dt2 / s.dy);
+    // FP32 momentum: the upwind differences carry sums of two face
+    // velocities, so dt2/dx and dt2/dy enter with the two halvings folded
+    const T dt2dx4 = T(0.25) * dt2dx, dt2dy4 = T(0.25) * dt2dy, dt2half = T(0.5) * dt2;
     const T rn = (T)s.rn;
 
     // forcing of this iteration: first continuity at t, second at t + dt
@@ -336,24 +339,44 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             if (__any(act_x && !b4)) val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
                 const T u13 = gg * hm1 * dt2dx * dsurf;
-                const T ur = T(0.50) * (uu1_0 + uu1_1);
-                const T ul = T(0.50) * (uu1_1 + uu1_2);
-                const T umr = ur * (u0 + u1) * T(0.50) + M::abs_(ur) * (u1 - u0) * T(0.50);
-                const T uml = ul * (u1 + u2) * T(0.50) + M::abs_(ul) * (u2 - u1) * T(0.50);
-                const T u11 = dt2dx * (umr - uml);
-                const T vu = T(0.50) * (vv1_1e + vv1_2e);
-                const T vd = T(0.50) * (vv1_1 + vv1_2);
-                const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
-                const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
-                const T u12 = (j != 0) ? dt2dy * (umu - umd) : T(0);
+                T u11, u12;
+                if constexpr (kF32) {
+                    // the halvings of the face velocities and of the upwind
+                    // products folded into dt2/(4 dx) and dt2/(4 dy)
+                    const T sr = uu1_0 + uu1_1, sl = uu1_1 + uu1_2;
+                    const T pr = sr * (u0 + u1) + M::abs_(sr) * (u1 - u0);
+                    const T pl = sl * (u1 + u2) + M::abs_(sl) * (u2 - u1);
+                    u11 = dt2dx4 * (pr - pl);
+                    const T su = vv1_1e + vv1_2e, sd = vv1_1 + vv1_2;
+                    const T pu = su * (u1e + u1) + M::abs_(su) * (u1 - u1e);
+                    const T pd = sd * (u1w + u1) + M::abs_(sd) * (u1w - u1);
+                    u12 = (j != 0) ? dt2dy4 * (pu - pd) : T(0);
+                } else {
+                    const T ur = T(0.50) * (uu1_0 + uu1_1);
+                    const T ul = T(0.50) * (uu1_1 + uu1_2);
+                    const T umr = ur * (u0 + u1) * T(0.50) + M::abs_(ur) * (u1 - u0) * T(0.50);
+                    const T uml = ul * (u1 + u2) * T(0.50) + M::abs_(ul) * (u2 - u1) * T(0.50);
+                    u11 = dt2dx * (umr - uml);
+                    const T vu = T(0.50) * (vv1_1e + vv1_2e);
+                    const T vd = T(0.50) * (vv1_1 + vv1_2);
+                    const T umu = vu * (u1e + u1) * T(0.50) + M::abs_(vu) * (u1 - u1e) * T(0.50);
+                    const T umd = vd * (u1w + u1) * T(0.50) + M::abs_(vd) * (u1w - u1) * T(0.50);
+                    u12 = (j != 0) ? dt2dy * (umu - umd) : T(0);
+                }
                 const T nf = map_on ? T(0.5) * (nc2 + nc1) : rn;
                 T ram = 0.00;
                 if (b4 && fr_x1) {
                     const T sqx = M::sqrt_(uu1_1 * uu1_1 + vva1 * vva1);
                     ram = M::div_(gg * nf * nf * sqx, pow1333(hm1));
                 }
-                const T up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * u1 + (-u11 - u12 - u13),
-                                     T(1.00) + dt2 * ram * T(0.50));
+                T up;
+                if constexpr (kF32) {
+                    const T a = dt2half * ram;
+                    up = M::div_((T(1.00) - a) * u1 + (-u11 - u12 - u13), T(1.00) + a);
+                } else {
+                    up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * u1 + (-u11 - u12 - u13),
+                                 T(1.00) + dt2 * ram * T(0.50));
+                }
                 if (b4) val = up;
             }
             umn = act_x ? val : T(0);
@@ -396,24 +419,42 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             if (__any(act_y && !b4)) val = wet ? coef * hv * M::sqrt_(c2 * hv) : T(0);
             if (__any(b4)) {
                 const T v13 = gg * hn1 * dt2dy * dsurf;
-                const T av = T(0.50) * (uu1_0 + uu1_0w);
-                const T bv = T(0.50) * (uu1_1 + uu1_1w);
-                const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
-                const T vnl = bv * (v1 + v2) * T(0.50) + M::abs_(bv) * (v2 - v1) * T(0.50);
-                const T v11 = (f != 0 && f != R) ? dt2dx * (vnr - vnl) : T(0);
-                const T pv = T(0.50) * (vv1_1e + vv1_1);
-                const T qv = T(0.50) * (vv1_1 + vv1_1w);
-                const T vnu = pv * (v1e + v1) * T(0.50) + M::abs_(pv) * (v1 - v1e) * T(0.50);
-                const T vnd = qv * (v1w + v1) * T(0.50) + M::abs_(qv) * (v1w - v1) * T(0.50);
-                const T v12 = dt2dy * (vnu - vnd);
+                T v11, v12;
+                if constexpr (kF32) {
+                    const T sa = uu1_0 + uu1_0w, sb = uu1_1 + uu1_1w;
+                    const T pr = sa * (v0 + v1) + M::abs_(sa) * (v1 - v0);
+                    const T pl = sb * (v1 + v2) + M::abs_(sb) * (v2 - v1);
+                    v11 = (f != 0 && f != R) ? dt2dx4 * (pr - pl) : T(0);
+                    const T sp = vv1_1e + vv1_1, sq = vv1_1 + vv1_1w;
+                    const T pu = sp * (v1e + v1) + M::abs_(sp) * (v1 - v1e);
+                    const T pd = sq * (v1w + v1) + M::abs_(sq) * (v1w - v1);
+                    v12 = dt2dy4 * (pu - pd);
+                } else {
+                    const T av = T(0.50) * (uu1_0 + uu1_0w);
+                    const T bv = T(0.50) * (uu1_1 + uu1_1w);
+                    const T vnr = av * (v0 + v1) * T(0.50) + M::abs_(av) * (v1 - v0) * T(0.50);
+                    const T vnl = bv * (v1 + v2) * T(0.50) + M::abs_(bv) * (v2 - v1) * T(0.50);
+                    v11 = (f != 0 && f != R) ? dt2dx * (vnr - vnl) : T(0);
+                    const T pv = T(0.50) * (vv1_1e + vv1_1);
+                    const T qv = T(0.50) * (vv1_1 + vv1_1w);
+                    const T vnu = pv * (v1e + v1) * T(0.50) + M::abs_(pv) * (v1 - v1e) * T(0.50);
+                    const T vnd = qv * (v1w + v1) * T(0.50) + M::abs_(qv) * (v1w - v1) * T(0.50);
+                    v12 = dt2dy * (vnu - vnd);
+                }
                 const T nf = map_on ? T(0.5) * (nc1w + nc1) : rn;
                 T ram = 0.00;
                 if (b4 && fr_y1) {
                     const T sqy = M::sqrt_(uua1 * uua1 + vv1_1 * vv1_1);
                     ram = M::div_(gg * nf * nf * sqy, pow1333(hn1));
                 }
-                const T up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * v1 + (-v11 - v12 - v13),
-                                     T(1.00) + dt2 * ram * T(0.50));
+                T up;
+                if constexpr (kF32) {
+                    const T a = dt2half * ram;
+                    up = M::div_((T(1.00) - a) * v1 + (-v11 - v12 - v13), T(1.00) + a);
+                } else {
+                    up = M::div_((T(1.00) - dt2 * ram * T(0.50)) * v1 + (-v11 - v12 - v13),
+                                 T(1.00) + dt2 * ram * T(0.50));
+                }
                 if (b4) val = up;
             }
             vnn = act_y ? val : T(0);
@@ -515,8 +556,13 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
                     const T pr = er * __builtin_amdgcn_rcpf(nb - sm);
                     if (cell_uniform((uint32_t)(q * C + j), (uint32_t)it.step_index) < pr) h2v = nb;
                 }
-                const double hrd = (double)sm + (double)er;
-                if (hrd < 0.0) { clip = -hrd; h2v = 0; }
+                // the exact update sm + er is negative iff sm < 0, or sm = 0 and
+                // er < 0 (|er| is at most half an ulp of sm); its FP64 value
+                // (exact: 48 bits suffice) is needed only then
+                if (sm < T(0) || (sm == T(0) && er < T(0))) {
+                    clip = -((double)sm + (double)er);
+                    h2v = 0;
+                }
             } else {
                 const TH hr = h3 - dt2h * (TH)rhs;
                 h2v = MH::max_(hr, TH(0.00));
