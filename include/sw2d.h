@@ -201,7 +201,11 @@ int sw2d_set_tile_rows(sw2d_handle *h, int tile_rows);
 
 /* Advance n loop iterations of the reference's call sequence (4798-4877):
  * flux, continuity(t), treat_error, continuity(t + dt), derived fields.
- * Asynchronous: returns once the work is queued. */
+ * Asynchronous: returns once the work is queued.  Groups of six iterations
+ * run as replays of a captured HIP graph; with SW2D_GRAPHS=0 in the
+ * environment when the handle (or batch) is created, every iteration is its
+ * own launch instead (for tools that trace launches one at a time), with the
+ * same results. */
 int sw2d_step(sw2d_handle *h, int n_iters);
 
 /* Run iterations while time + dt <= tmax (the loop condition of 4798), at
@@ -451,6 +455,8 @@ typedef struct sw2d_server_stats {
  * height of at most 48, else 32; results are those of any fixed height, the
  * volume ledger to rounding).
  * slice_iters: iterations per batch step between admissions (0 = 60).
+ * The server's batch interleaves the running requests' tiles block by block
+ * (sw2d_batch_set_schedule order 1).
  * max_active_cells: admission also stops while the running scenarios hold
  * this many cells (0 = no limit; a request is always admitted onto an idle
  * GPU).  With enough cells in flight to fill the GPU, admitting more only

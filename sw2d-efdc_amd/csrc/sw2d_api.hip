@@ -333,6 +333,7 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     }
 
     sw2d_handle *h = new sw2d_handle();
+    h->graph_on = sw2d::graphs_enabled();
     h->batch = batch;
     h->grid = *grid;
     h->prm = *params;

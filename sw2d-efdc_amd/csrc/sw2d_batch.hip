@@ -242,6 +242,7 @@ int sw2d_batch_create(int device, int precision, size_t pool_bytes, int tile_row
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return bfail(nullptr, SW2D_E_NODEV, "no HIP device");
     if (device < 0 || device >= ndev) return bfail(nullptr, SW2D_E_NODEV, "bad device ordinal");
     sw2d_batch *b = new sw2d_batch();
+    b->graph_on = sw2d::graphs_enabled();
     b->device = device;
     b->precision = precision;
     b->H = tile_rows > 0 ? tile_rows : 32;

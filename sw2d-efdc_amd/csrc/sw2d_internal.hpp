@@ -19,10 +19,22 @@
 #include <string>
 #include <vector>
 
+#include <stdlib.h>
+
 #include "../../include/sw2d.h"
 #include "sw2d_pool.hpp"
 
 namespace sw2d {
+
+// SW2D_GRAPHS=0 in the environment when a handle or batch is created: its
+// iterations are launched one by one instead of as replays of the captured
+// six-launch graph (for tools that trace launches one at a time); the
+// results are the same either way
+inline bool graphs_enabled()
+{
+    const char *e = getenv("SW2D_GRAPHS");
+    return !(e && e[0] == '0');
+}
 
 // pointer into global memory (address space 1 of the AMDGPU target)
 template <typename T>

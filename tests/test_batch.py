@@ -316,3 +316,41 @@ def test_batch_autotune_searches_the_schedule(pkg, cases):
         # the state does not depend on the tiling or the schedule
         assert_same(members[i], alone[i], f"tuned member {i}")
     batch.close()
+
+
+def test_eager_launches_equal_graph_replays(pkg, cases, monkeypatch):
+    """SW2D_GRAPHS=0 at creation: every iteration is its own launch instead of
+    a replay of the six-launch graph; a stand-alone case and a batch give the
+    same state and ledger as with graph replays."""
+    spec = cases.ensemble_scenario(5, side=160)
+
+    def run(batch):
+        if batch:
+            b = pkg.Batch(precision="fp32", pool_bytes=128 << 20)
+            m = b.case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning)
+            m.set_rain(spec.rain, spec.dtrain)
+            m.set_outlet(*spec.outlet)
+            b.attach(m, t_end=spec.t_end)
+            for _ in range(10):
+                b.step(61)
+            out = (m.field("h"), m.balance()["volume"], m.balance()["q_peak"])
+            b.close()
+            return out
+        c = pkg.Case(spec.z, spec.h0, spec.cellsize, dt=spec.dt, manning=spec.manning,
+                     precision="fp64")
+        c.set_rain(spec.rain, spec.dtrain)
+        c.set_outlet(*spec.outlet)
+        for _ in range(10):
+            c.step(61)
+        out = (c.field("h"), c.balance()["volume"], c.balance()["q_peak"])
+        c.close()
+        return out
+
+    for batch in (False, True):
+        monkeypatch.delenv("SW2D_GRAPHS", raising=False)
+        g = run(batch)
+        monkeypatch.setenv("SW2D_GRAPHS", "0")
+        e = run(batch)
+        monkeypatch.delenv("SW2D_GRAPHS")
+        assert np.array_equal(g[0], e[0]) and g[1] == e[1] and g[2] == e[2]
+        assert g[1] > 0
