@@ -43,6 +43,7 @@ import numpy as np
 ROOT = pathlib.Path(__file__).resolve().parent
 METRIC = "cell-updates/s + % HBM roofline; p50/p99 scenario latency vs plain FP64 GPU"
 SERVE_TRAFFIC = ROOT / "profiles" / "serving_traffic.json"
+SERVE_ISSUE = ROOT / "profiles" / "serving_issue.json"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -668,6 +669,19 @@ def serve_traffic(precision):
     return t.get("bytes_per_cell_update", {}).get(precision)
 
 
+def serve_issue(precision):
+    """VALU wave-instructions per cell-update of the served kernel (rocprofv3
+    SQ counters over the serving session, committed under profiles/) and the
+    VALU issue peak; None when no measurement of this form is recorded."""
+    try:
+        t = json.load(open(SERVE_ISSUE))
+    except (OSError, ValueError):
+        return None
+    per = t.get("valu_wave_instructions_per_cell_update", {}).get(precision)
+    peak = t.get("valu_issue_peak_wave_instructions_per_s")
+    return (per, peak) if per and peak else None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -750,6 +764,16 @@ def main():
     rf = sv["roofline"]
     traffic_bpc = serve_traffic(args.serve_precision)
     cells_per_launch = (sv["value"] or 0.0) * sv["wall_s"] / max(sv["launches"], 1)
+    issue = serve_issue(args.serve_precision)
+    valu_issue = None
+    if issue and sv["value"]:
+        per, peak = issue
+        valu_issue = {"wave_instructions_per_cell_update": per,
+                      "peak_wave_instructions_per_s": peak,
+                      "bound_cell_updates_per_s": peak / per,
+                      "frac": sv["value"] / (peak / per),
+                      "source": "rocprofv3 SQ_INSTS_VALU over a serving session; "
+                                "profiles/serving_issue.json"}
     out = {
         "metric": METRIC,
         "value": sv["value"],
@@ -790,6 +814,10 @@ def main():
             "kernel_avg_us": rf["kernel_avg_us"],
             "stream_copy_GBs": copy_gbs,
             "frac_of_stream_copy": (rf["achieved"] / copy_gbs) if rf["achieved"] else None,
+            # the bound the kernel actually meets: VALU issue (one wave64
+            # instruction per SIMD per 4 cycles) at its measured instruction
+            # count per cell-update
+            "valu_issue": valu_issue,
         },
         "serving": sv,
     }
