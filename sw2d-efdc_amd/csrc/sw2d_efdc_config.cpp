@@ -502,13 +502,22 @@ bool write_efdc_config(const std::string &path, const EfdcConfig &c, std::string
         out.append(s, tail, std::string::npos);
         s.swap(out);
     }
-    FILE *fp = fopen(path.c_str(), "wb");
+    // written beside and renamed over the file, so a failed write leaves the
+    // previous file whole
+    const std::string tmp = path + ".tmp";
+    FILE *fp = fopen(tmp.c_str(), "wb");
     if (!fp) {
-        if (err) *err = "cannot write " + path;
+        if (err) *err = "cannot write " + tmp;
         return false;
     }
-    const bool ok = fwrite(s.data(), 1, s.size(), fp) == s.size();
-    return (fclose(fp) == 0) && ok;
+    bool ok = fwrite(s.data(), 1, s.size(), fp) == s.size();
+    ok = (fclose(fp) == 0) && ok;
+    if (ok && rename(tmp.c_str(), path.c_str()) != 0) ok = false;
+    if (!ok) {
+        remove(tmp.c_str());
+        if (err) *err = "cannot write " + path;
+    }
+    return ok;
 }
 
 }  // namespace sw2dio
