@@ -430,6 +430,50 @@ def _esri(path, a, cellsize, nodata, fmt, xll=0.0, yll=0.0):
             f.write(" ".join(fmt(v) for v in row) + "\n")
 
 
+# grid_make_input.dat: the grid generator's controls, in the file's schema
+GRID_MAKE_INPUT = """SOLUTION ITERATIONS AND SMOOTHING INFORMATION
+ITRXM: 150
+ITRHM: 160
+ITRKM: 170
+ITRGM: 180
+NDEPSM: 2000
+NDEPSMF: 1
+DEPMIN: 0.5
+DDATADJ: 0
+PARAMETERS AND CONVERGENCE CRITERIA
+RPX: 1.7
+RPK: 1.6
+RPH: 1.5
+RSQXM: 1.E-10
+RSQKM: 2.E-11
+RSQKIM: 3.E-12
+RSQHM: 4.E-13
+RSQHIM: 5.E-14
+RSQHJM: 6.E-15
+COORDINATE SHIFT PARAMETERS
+XSHIFT: 10.
+YSHIFT: 20.
+HSCALE: 1.
+RKJDKI: 1.
+ANGORO: 4.0
+INTERPOLATION SWITCHES
+ISIRKI: 1
+JSIRKI: 0
+ISIHIHJ: 1
+JSIHIHJ: 0
+DEPTH INTERPOLATION SWITCHES
+ISIDEP: 1
+CDEP: 2.
+RADM: 3.
+ISIDPTYP: 2
+SURFELV: 9.5
+ISVEG: 0
+NVEGDAT: 0
+NVEGTYP: 0
+ZROUGH: 0.0005
+VERBOSE: 1"""
+
+
 def efdc_card_template():
     """A small EFDC.INP card template (CRLF lines, as EFDC's own): cards C1 to
     C91 with C11A and C71A, each a title line, a comment, a header line, one
@@ -449,7 +493,7 @@ def efdc_card_template():
 def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=None,
              efdc=False, basetime=86400.0, nlayers=1, evaporation=None, INT=0.0, INF=0.0,
              LWL=0.0, EV_WL_min=0.0, outputs=(1, 1, 1, 1, 1), efdc_values=None, met=None,
-             efdc_points=((15.0, 25.0),), tctmsr=1.0, efdc_template=None):
+             efdc_points=((15.0, 25.0),), tctmsr=1.0, efdc_template=None, grid_make=None):
     """Write db/dir.txt and db/<name>/ for a fixed-step spec in the reference's
     layout.  probes: (x, y) metres for coord_out.dat; sources: (xs, ys, Q, dtoq);
     evaporation: (albedo, T[], Rg[]) with one value per rain interval.
@@ -457,7 +501,8 @@ def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=N
     wind[(speed, direction)]) for atm_press.dat, rel_humidity.dat and
     wind_velocities.dat (constant series when None); efdc_points, tctmsr:
     efdc_coord_out.dat; efdc_template: bytes of EFDC/input/template/EFDC.INP
-    (efdc_card_template() when None)."""
+    (efdc_card_template() when None); grid_make: text of grid_make_input.dat
+    (GRID_MAKE_INPUT when None)."""
     import os
     base = os.path.join(root, "db", name)
     for d in ("SW2D/input", "SW2D/output", "EFDC/input", "EFDC/output"):
@@ -526,4 +571,6 @@ def write_db(spec, root, name, *, dkout=600.0, dpout=600.0, probes=(), sources=N
         os.makedirs(tdir, exist_ok=True)
         with open(os.path.join(tdir, "EFDC.INP"), "wb") as f:
             f.write(efdc_template if efdc_template is not None else efdc_card_template())
+        with open(os.path.join(base, "grid_make_input.dat"), "w") as f:
+            f.write(grid_make if grid_make is not None else GRID_MAKE_INPUT)
     return base

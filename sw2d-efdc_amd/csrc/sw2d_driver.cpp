@@ -9,19 +9,23 @@
 // EFDC/input/QSER.INP (couple_BC_efdc + write_efdc_QSER, with the
 // reference's one-output lag of the velocities).  With the coupling on it
 // also writes EFDC's series and initial-condition inputs (WSER, ASER, SHOW,
-// DYE/SALT/TEMP.INP, DSER/TSER/SSER) and EFDC/input/EFDC.INP: a copy of
-// EFDC/input/template/EFDC.INP made before the run (4281-4289) whose cards
-// C6-C11, C14, C22-C26, C71-C73, C84 and C87 are rewritten with the first
-// output (4924-4933).  GEFDC/Gorp and the mpirun of the Fortran lake model
-// are not part of this build.
+// DYE/SALT/TEMP.INP, DSER/TSER/SSER), the grid generator's inputs
+// (EFDC/input/gefdc/: CELL.INP, gridext.inp, depdat.inp, gefdc.inp) and
+// EFDC/input/EFDC.INP: a copy of EFDC/input/template/EFDC.INP made before
+// the run (4281-4289) whose cards C6-C11, C14, C22-C26, C71-C73, C84 and C87
+// are rewritten with the first output (4924-4933).  Running the grid
+// generator, the Gorp domain decomposition and the mpirun of the Fortran lake
+// model are not part of this build.
 //
 // usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32|mixed]
 //                  [--device N] [--autotune] [--parse-only] [--efdc-inputs]
 // --efdc-inputs writes the EFDC inputs known before the run (WSER, ASER,
-// SHOW, DYE/SALT/TEMP.INP, EFDC.INP) and stops; no GPU is used.
+// SHOW, DYE/SALT/TEMP.INP, EFDC.INP, the grid generator's inputs) and stops;
+// no GPU is used.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
 
 #include <atomic>
 #include <condition_variable>
@@ -238,6 +242,30 @@ sw2dio::EfdcConfig efdc_config(const sw2dio::Case &c, double timmax, int water)
     return e;
 }
 
+// The grid generator's inputs (make_gefdc_inputs at 4472: CELL.INP,
+// gridext.inp, depdat.inp, gefdc.inp) from mask.dat, the smoothed bed and
+// the case's grid_make_input.dat, into EFDC/input/gefdc/, the folder to run
+// the grid generator in (the reference writes them to its working directory
+// and runs it there, 4472-4535).
+bool write_gefdc(const sw2dio::Case &c, const std::vector<double> &z)
+{
+    sw2dio::GefdcParams gp;
+    std::string err;
+    if (!sw2dio::read_grid_make_input(c.dir + "/grid_make_input.dat", gp, &err)) {
+        printf("%s\n", err.c_str());
+        return false;
+    }
+    const std::string dir = c.dir + "/EFDC/input/gefdc/";
+    mkdir(dir.c_str(), 0755);
+    int nd = 0;
+    if (!sw2dio::write_gefdc_inputs(dir, c.mask_hdr.cols, c.mask_hdr.rows, c.mask_hdr.cellsize, c.mask,
+                                    z, gp, &nd, &err)) {
+        printf(" %s\n", err.c_str());
+        return false;
+    }
+    return true;
+}
+
 // records come back from sw2d_couple_bc as S, N, E, W; QSER.INP and C24
 // list them N, S, E, W
 const int kind_of_edge[4] = {1, 0, 2, 3};
@@ -372,6 +400,9 @@ int run_case(const Options &opt, const std::string &name)
             printf(" cannot write %s/EFDC/input/SHOW.INP\n", c.dir.c_str());
             return 1;
         }
+        std::vector<double> zs = c.dem.v;
+        sw2d_lake_catchment_smooth(R, C, c.mask.data(), zs.data());  // 4460
+        if (!write_gefdc(c, zs)) return 1;
         // EFDC.INP: the first output's coupling records depend only on
         // mask.dat (couple_BC_efdc at output 0 takes the initial depths and
         // zero velocities, 4902), so its cards are known before the run
@@ -395,7 +426,10 @@ int run_case(const Options &opt, const std::string &name)
     }
 
     std::vector<double> z = c.dem.v;
-    if (s.efdc == 1) sw2d_lake_catchment_smooth(R, C, c.mask.data(), z.data());  // 4460
+    if (s.efdc == 1) {
+        sw2d_lake_catchment_smooth(R, C, c.mask.data(), z.data());  // 4460
+        if (!write_gefdc(c, z)) return 1;
+    }
 
     sw2d_grid_desc grid{R, C, c.dem.xll, c.dem.yll, c.dem.cellsize, c.dem.nodata};
     sw2d_params prm;

@@ -157,4 +157,32 @@ struct EfdcConfig {
 // one of the cards or their successors.
 bool write_efdc_config(const std::string &path, const EfdcConfig &c, std::string *err = nullptr);
 
+// grid_make_input.dat of a coupled case (the grid generator's controls,
+// read at 2944-2991)
+struct GefdcParams {
+    int ITRXM = 0, ITRHM = 0, ITRKM = 0, ITRGM = 0, NDEPSM = 0, NDEPSMF = 0, DDATADJ = 0;
+    float DEPMIN = 0, RPX = 0, RPK = 0, RPH = 0;
+    double RSQXM = 0, RSQKM = 0, RSQKIM = 0, RSQHM = 0, RSQHIM = 0, RSQHJM = 0;
+    float XSHIFT = 0, YSHIFT = 0, HSCALE = 0, RKJDKI = 0, ANGORO = 0;
+    int ISIRKI = 0, JSIRKI = 0, ISIHIHJ = 0, JSIHIHJ = 0, ISIDEP = 0;
+    float CDEP = 0, RADM = 0;
+    int ISIDPTYP = 0;
+    float SURFELV = 0;
+    int ISVEG = 0, NVEGDAT = 0, NVEGTYP = 0;
+    double ZROUGH = 0;
+    int VERBOSE = 0;
+};
+bool read_grid_make_input(const std::string &path, GefdcParams &p, std::string *err = nullptr);
+
+// CELL.INP cell types (0, 5 lake, 9 boundary) of a rows x cols mask, rows
+// from the north (make_gefdc_inputs, 2877-3196)
+std::vector<unsigned char> gefdc_cell_types(int cols, int rows, const std::vector<int> &mask);
+
+// CELL.INP, gridext.inp, depdat.inp and gefdc.inp into directory `dir`
+// (ending in '/') from the mask and the bed (rows x cols, north row first,
+// after lake_catchment_smooth); *ndepdat receives the lake-cell count
+bool write_gefdc_inputs(const std::string &dir, int cols, int rows, double res,
+                        const std::vector<int> &mask, const std::vector<double> &bed,
+                        const GefdcParams &p, int *ndepdat = nullptr, std::string *err = nullptr);
+
 }  // namespace sw2dio

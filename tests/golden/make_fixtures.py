@@ -9,7 +9,9 @@ not part of the repository, so no flow results exist to compare against.
 The case's EFDC card template (EFDC/input/template/EFDC.INP, the file the
 coupled run copies and rewrites cards of) and its EFDC time-series locations
 (SW2D/input/efdc_coord_out.dat) are stored as they are, the template
-gzip-compressed: peri_lake_EFDC.INP.gz, peri_lake_efdc_coord_out.dat.
+gzip-compressed: peri_lake_EFDC.INP.gz, peri_lake_efdc_coord_out.dat; so is
+its grid generator's control file (grid_make_input.dat):
+peri_lake_grid_make_input.dat.
 """
 import gzip
 import pathlib
@@ -47,6 +49,7 @@ def main(ref="/root/reference"):
     with gzip.GzipFile(HERE / "peri_lake_EFDC.INP.gz", "wb", mtime=0) as g:
         g.write(raw)
     shutil.copyfile(case / "efdc_coord_out.dat", HERE / "peri_lake_efdc_coord_out.dat")
+    shutil.copyfile(root / "grid_make_input.dat", HERE / "peri_lake_grid_make_input.dat")
     print(len(raw), "bytes of EFDC.INP")
 
 
