@@ -120,8 +120,8 @@ const char *sw2d_last_error(const sw2d_handle *h);
  * initialisation: init_inf + initiald (SW2D-EFDC.cu:29-91, launched at
  * 4763-4766) after the uploads of 4677-4729.  Cells with z == nodata are
  * inactive.  The library owns all device memory of the handle.  Grids of
- * 2^31 or more cells (cols padded to a multiple of 16) are refused with
- * SW2D_E_ARG: cell offsets on the device are 32-bit. */
+ * 2^29 or more cells (cols padded to a multiple of 16) are refused with
+ * SW2D_E_ARG: offsets on the device are 32-bit (byte offsets of doubles). */
 int sw2d_create(const sw2d_grid_desc *grid, const double *z, const double *h0,
                 const sw2d_params *params, sw2d_handle **out);
 

@@ -312,13 +312,15 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     if (params->tile_rows < 0 || params->tile_rows > 512)
         return fail(nullptr, SW2D_E_ARG, "tile_rows must be in [0, 512]");
     {
-        // cell offsets in the kernels are 32-bit: the padded grid (pitch a
-        // multiple of 16) and the plain layout (R*C + C/2 plus 2C + 4 of tail)
-        // must stay below 2^31 elements
+        // offsets in the kernels are 32-bit: byte offsets of doubles in the
+        // padded grid (pitch a multiple of 16; the fused kernel addresses its
+        // arrays through buffer resources) and element offsets of the plain
+        // layout (R*C + C/2 plus 2C + 4 of tail) must fit, so both stay below
+        // 2^29 cells
         const long long R = grid->rows, C = grid->cols;
         const long long padded = R * ((C + 15) / 16 * 16), flat = R * C + C / 2 + 2 * C + 4;
-        if (padded >= (1LL << 31) || flat >= (1LL << 31))
-            return fail(nullptr, SW2D_E_ARG, "grid too large: rows * cols must stay below 2^31 cells");
+        if (padded >= (1LL << 29) || flat >= (1LL << 29))
+            return fail(nullptr, SW2D_E_ARG, "grid too large: rows * cols must stay below 2^29 cells");
     }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)

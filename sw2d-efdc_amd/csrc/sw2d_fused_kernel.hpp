@@ -49,6 +49,44 @@ __device__ __forceinline__ bool west_bit(bool p, int lane)
     return lane > 0 && ((m >> (lane - 1)) & 1ULL);
 }
 
+// One scenario array, read and written at 32-bit element offsets.  Buf =
+// true: through a buffer resource (base and range in scalar registers, a
+// 32-bit byte offset per lane: no per-lane 64-bit address arithmetic; cell
+// offsets stay below 2^29, which the host enforces, so byte offsets of
+// doubles fit).  Buf = false: a global-address-space pointer (SGPR base +
+// 64-bit VGPR offset).  Measured: buffers make the FP32 form's launches
+// 1-2% faster and the FP64 and mixed forms' 1-2% slower, so only the FP32
+// form uses them.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <typename V, bool Buf> struct Arr;
+template <typename V> struct Arr<V, false> {
+    gptr<V> p;
+    __device__ explicit Arr(const void *q) : p((gptr<V>)const_cast<void *>(q)) {}
+    __device__ __forceinline__ V load(unsigned off) const { return p[off]; }
+    __device__ __forceinline__ void store(unsigned off, V v) const { p[off] = v; }
+};
+template <typename V> struct Arr<V, true> {
+    __amdgpu_buffer_rsrc_t r;
+    // range: the whole 32-bit offset space; 0x00020000: the 32-bit data
+    // format (a buffer with the invalid format reads zeros)
+    __device__ explicit Arr(const void *q)
+        : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(q), (short)0, -1, 0x00020000)) {}
+    __device__ __forceinline__ V load(unsigned off) const
+    {
+        if constexpr (sizeof(V) == 4)
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(off * 4u), 0, 0));
+        else
+            return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(off * 8u), 0, 0));
+    }
+    __device__ __forceinline__ void store(unsigned off, V v) const
+    {
+        if constexpr (sizeof(V) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, (int)(off * 4u), 0, 0);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)(off * 8u), 0, 0);
+    }
+};
+
 constexpr int kMaxTileRows = 512;
 
 // F: the features (kFeat*) this instantiation handles.  A feature missing
@@ -105,16 +143,11 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         }
     }
     const int ib = k6 & 1;
-    // the descriptor's pointers are global memory: saying so lets the
-    // compiler address them as SGPR base + VGPR offset (global_load) instead
-    // of 64-bit flat addresses in VGPRs
-    const gptr<const TH> h_in = (gptr<const TH>)d.st[ib][0];
-    const gptr<const T> um_in = (gptr<const T>)d.st[ib][1];
-    const gptr<const T> vn_in = (gptr<const T>)d.st[ib][2];
-    const gptr<TH> h_out = (gptr<TH>)d.st[ib ^ 1][0];
-    const gptr<T> um_out = (gptr<T>)d.st[ib ^ 1][1];
-    const gptr<T> vn_out = (gptr<T>)d.st[ib ^ 1][2];
-    const gptr<const double> zg = (gptr<const double>)d.z;
+    // the state arrays (buffer resources in the FP32 form, see Arr); the
+    // Manning class bytes, read only with a map, through a global pointer
+    constexpr bool kBuf = sizeof(T) == 4 && sizeof(TH) == 4;
+    const Arr<TH, kBuf> h_out(d.st[ib ^ 1][0]);
+    const Arr<T, kBuf> um_out(d.st[ib ^ 1][1]), vn_out(d.st[ib ^ 1][2]);
     const gptr<const uint8_t> clsg = (gptr<const uint8_t>)d.cls;
 
     if (tile >= d.ntiles) {
@@ -221,13 +254,16 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     // values (0, inactive), so the loads need no branch.
     const int kbeg = r0 - 3, kend = r1 + 3;
     const int jc = colin ? j : (j < 0 ? 0 : C - 1);
+    const Arr<const TH, kBuf> h_in(d.st[ib][0]);
+    const Arr<const T, kBuf> um_in(d.st[ib][1]), vn_in(d.st[ib][2]);
+    const Arr<const double, kBuf> zg(d.z);
     auto load_row = [&](int k, TH &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
-        const int off = kc * P + jc;
-        const TH lh = h_in[off];
-        const T lu = um_in[off], lv = vn_in[off];
-        const double lz = zg[off];
+        const unsigned off = (unsigned)(kc * P + jc);
+        const TH lh = h_in.load(off);
+        const T lu = um_in.load(off), lv = vn_in.load(off);
+        const double lz = zg.load(off);
         hh = in ? lh : TH(0);
         uu = in ? lu : T(0);
         vv = in ? lv : T(0);
@@ -570,10 +606,10 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
             }
             if (!a3) h2v = 0.00;
             if (colout && q >= r0 && q < r1) {
-                const long off = (long)q * P + j;
-                h_out[off] = h2v;
-                um_out[off] = umpP;
-                vn_out[off] = vnpP;
+                const unsigned off = (unsigned)(q * P + j);
+                h_out.store(off, h2v);
+                um_out.store(off, umpP);
+                vn_out.store(off, vnpP);
                 if (a3) {
                     acc_vol += (double)h2v;
                     if constexpr (kF32) acc_src += (double)(ql + rr * Pc - E);  // x dt2 below

@@ -82,8 +82,9 @@ def test_bench_runs_on_one_gpu_only():
 
 
 def test_create_rejects_grids_beyond_32bit_offsets(pkg):
-    """Cell offsets in the kernels are 32-bit: a grid of 2^31 or more cells
-    (padded) is refused before any array is read or any device work."""
+    """Offsets in the kernels are 32-bit (byte offsets of doubles in the
+    fused kernel): a grid of 2^29 or more cells (padded) is refused before
+    any array is read or any device work."""
     sw = pkg.sw2d
     L = sw.lib()
     z = np.zeros(4)
@@ -93,5 +94,5 @@ def test_create_rejects_grids_beyond_32bit_offsets(pkg):
     g = sw.GridDesc(46341, 46341, 0.0, 0.0, 10.0, -9999.0)   # 2.1e9 cells
     assert L.sw2d_create(ct.byref(g), sw._dp(z), sw._dp(z), ct.byref(p), ct.byref(out)) == -1
     assert b"too large" in L.sw2d_last_error(None)
-    g = sw.GridDesc(65536, 32760, 0.0, 0.0, 10.0, -9999.0)   # pitch 32768: 2^31 padded cells
+    g = sw.GridDesc(16384, 32760, 0.0, 0.0, 10.0, -9999.0)   # pitch 32768: 2^29 padded cells
     assert L.sw2d_create(ct.byref(g), sw._dp(z), sw._dp(z), ct.byref(p), ct.byref(out)) == -1
