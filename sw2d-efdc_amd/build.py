@@ -25,7 +25,7 @@ HIP_SOURCES = [("sw2d_api.hip", []), ("sw2d_fused64.hip", []),
                ("sw2d_fusedmix.hip", ["-ffp-contract=on"]), ("sw2d_plain.hip", []),
                ("sw2d_batch.hip", []), ("sw2d_server.hip", []), ("sw2d_output.hip", []),
                ("sw2d_host.cpp", [])]
-DRIVER_SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp", "sw2d_efdc_config.cpp", "sw2d_gefdc.cpp"]
+DRIVER_SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp", "sw2d_efdc_config.cpp", "sw2d_gefdc.cpp", "sw2d_lorp.cpp"]
 ARCH = os.environ.get("SW2D_OFFLOAD_ARCH", "gfx950")
 
 

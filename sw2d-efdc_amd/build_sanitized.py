@@ -22,7 +22,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp", "sw2d_efdc_config.cpp", "sw2d_gefdc.cpp",
+SOURCES = ["sw2d_driver.cpp", "sw2d_io.cpp", "sw2d_efdc_config.cpp", "sw2d_gefdc.cpp", "sw2d_lorp.cpp",
            "sw2d_host.cpp"]
 FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
          "-fno-sanitize-recover=undefined", "-std=c++17", "-Wall", "-pthread"]

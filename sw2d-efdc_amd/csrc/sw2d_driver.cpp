@@ -10,18 +10,20 @@
 // reference's one-output lag of the velocities).  With the coupling on it
 // also writes EFDC's series and initial-condition inputs (WSER, ASER, SHOW,
 // DYE/SALT/TEMP.INP, DSER/TSER/SSER), the grid generator's inputs
-// (EFDC/input/gefdc/: CELL.INP, gridext.inp, depdat.inp, gefdc.inp) and
-// EFDC/input/EFDC.INP: a copy of EFDC/input/template/EFDC.INP made before
+// (EFDC/input/gefdc/: CELL.INP, gridext.inp, depdat.inp, gefdc.inp), EFDC's
+// domain decomposition EFDC/input/LORP.INP and EFDC/input/EFDC.INP: a copy of EFDC/input/template/EFDC.INP made before
 // the run (4281-4289) whose cards C6-C11, C14, C22-C26, C71-C73, C84 and C87
 // are rewritten with the first output (4924-4933).  Running the grid
-// generator, the Gorp domain decomposition and the mpirun of the Fortran lake
-// model are not part of this build.
+// generator and the mpirun of the Fortran lake model are not part of this
+// build.
 //
 // usage: sw2d-efdc [--root DIR] [--kernel fused|plain] [--precision fp64|fp32|mixed]
 //                  [--device N] [--autotune] [--parse-only] [--efdc-inputs]
+//        sw2d-efdc --lorp CELL.INP NPROC TYPE OUT
 // --efdc-inputs writes the EFDC inputs known before the run (WSER, ASER,
-// SHOW, DYE/SALT/TEMP.INP, EFDC.INP, the grid generator's inputs) and stops;
-// no GPU is used.
+// SHOW, DYE/SALT/TEMP.INP, EFDC.INP, the grid generator's inputs, LORP.INP)
+// and stops; no GPU is used.  --lorp writes only the decomposition of a
+// CELL.INP for NPROC ranks and LORP type TYPE (1 h, 2 v, 3 r) to OUT.
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
@@ -103,6 +105,8 @@ struct Options {
     bool autotune = false;
     bool parse_only = false;
     bool efdc_inputs = false;
+    // --lorp CELL.INP NPROC TYPE OUT: only the domain decomposition file
+    std::vector<std::string> lorp;
     // fault injection for tests: once this many outputs are out, the next
     // step request asks for a negative iteration count, which the library
     // rejects as it would any failed step (-1: off)
@@ -127,6 +131,10 @@ bool parse_args(int argc, char **argv, Options &o)
         else if (a == "--autotune") o.autotune = true;
         else if (a == "--parse-only") o.parse_only = true;
         else if (a == "--efdc-inputs") o.efdc_inputs = true;
+        else if (a == "--lorp" && i + 4 < argc) {
+            o.lorp.assign(argv + i + 1, argv + i + 5);
+            i += 4;
+        }
         else if (a == "--fail-step-after-outputs" && next(v)) o.fail_after_outputs = atol(v.c_str());
         else {
             fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -260,6 +268,17 @@ bool write_gefdc(const sw2dio::Case &c, const std::vector<double> &z)
     int nd = 0;
     if (!sw2dio::write_gefdc_inputs(dir, c.mask_hdr.cols, c.mask_hdr.rows, c.mask_hdr.cellsize, c.mask,
                                     z, gp, &nd, &err)) {
+        printf(" %s\n", err.c_str());
+        return false;
+    }
+    // the domain decomposition for setup.dat's number of EFDC ranks and LORP
+    // type (the reference runs its partitioner on CELL.INP and renames the
+    // chosen file, 4480-4507)
+    const std::vector<double> &b = c.setup.efdc_block;
+    const std::vector<unsigned char> types =
+        sw2dio::gefdc_cell_types(c.mask_hdr.cols, c.mask_hdr.rows, c.mask);
+    if (!sw2dio::write_lorp(c.dir + "/EFDC/input/LORP.INP", types, c.mask_hdr.cols, c.mask_hdr.rows,
+                            (int)b[sw2dio::kNProcessors], (int)b[sw2dio::kLorpType], &err)) {
         printf(" %s\n", err.c_str());
         return false;
     }
@@ -663,6 +682,18 @@ int main(int argc, char **argv)
 {
     Options opt;
     if (!parse_args(argc, argv, opt)) return 2;
+    if (!opt.lorp.empty()) {
+        std::vector<unsigned char> types;
+        int cols = 0, rows = 0;
+        std::string err;
+        if (!sw2dio::read_cell_types(opt.lorp[0], types, cols, rows, &err) ||
+            !sw2dio::write_lorp(opt.lorp[3], types, cols, rows, atoi(opt.lorp[1].c_str()),
+                                atoi(opt.lorp[2].c_str()), &err)) {
+            printf("%s\n", err.c_str());
+            return 1;
+        }
+        return 0;
+    }
     std::vector<std::string> names;
     const std::string err = sw2dio::read_dir_list(opt.root, names);
     if (!err.empty()) {

@@ -185,4 +185,17 @@ bool write_gefdc_inputs(const std::string &dir, int cols, int rows, double res,
                         const std::vector<int> &mask, const std::vector<double> &bed,
                         const GefdcParams &p, int *ndepdat = nullptr, std::string *err = nullptr);
 
+// EFDC's domain decomposition file (LORP.INP) for `nproc` ranks: the
+// partition the reference's Gorp tool finds on these cell types (rows x
+// cols, north row first, as gefdc_cell_types gives them) and the file it
+// writes, for LORP type 1 (h: nproc columns of tiles), 2 (v: nproc rows) or
+// 3 (r: the best grid of tiles); called at 4480-4507 of SW2D-EFDC.cu
+bool write_lorp(const std::string &path, const std::vector<unsigned char> &types, int cols, int rows,
+                int nproc, int kind, std::string *err = nullptr);
+
+// cell types of a CELL.INP file, rows from the north, read the way the
+// decomposition tool reads them (Gorp.cpp:17-60)
+bool read_cell_types(const std::string &path, std::vector<unsigned char> &types, int &cols, int &rows,
+                     std::string *err = nullptr);
+
 }  // namespace sw2dio
