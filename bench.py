@@ -603,15 +603,24 @@ def pond_break_leg(pkg, cases, side, sample=30.0):
 
 
 def cpu_model():
-    """The host CPU's model name (from /proc/cpuinfo), or None."""
+    """The host CPU's model name with its family / model / stepping numbers
+    (from /proc/cpuinfo: a virtual machine may report only "AMD EPYC", the
+    numbers still tell the generation), or None."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break
+                k, _, v = line.partition(":")
+                info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return None
+        return None
+    name = info.get("model name")
+    if not name:
+        return None
+    ids = [f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info]
+    return f"{name} ({', '.join(ids)})" if ids else name
 
 
 def cpu_baseline(cases, seconds, threads, hours, config0=None):

@@ -452,7 +452,7 @@ typedef struct sw2d_server_stats {
 /* tile_rows: tile height of every request that names none (0 = adaptive:
  * after each admission or retirement the running requests are retiled to the
  * height that fills whole rounds of wave slots, the fewest rounds with a
- * height of at most 48, else 32; results are those of any fixed height, the
+ * height of at most 80, else 32; results are those of any fixed height, the
  * volume ledger to rounding).
  * slice_iters: iterations per batch step between admissions (0 = 60).
  * The server's batch interleaves the running requests' tiles block by block

@@ -187,7 +187,9 @@ namespace sw2d {
 
 // Round-filling tile height for the members still running (a member with a
 // height of its own keeps it): the fewest whole rounds of wave slots whose
-// height is at most 48, else 32 when even six rounds need taller tiles.  The
+// height is at most 80, else 32 when even six rounds need taller tiles (on
+// the serving mix two rounds at H ~ 55 beat three at H ~ 37 by 1%: fewer halo
+// rows per output row).  The
 // running members are retiled to it and new members start at it.  A member
 // whose retile finds no pool memory keeps its height (members of one launch
 // may differ in height).  *retiled counts the members moved.
@@ -213,7 +215,7 @@ int batch_fit_tiles(sw2d_batch *b, int *tile_rows, int *retiled)
         });
         int best = 32;
         for (auto it = fit.rbegin(); it != fit.rend(); ++it)
-            if (*it <= 48) { best = *it; break; }
+            if (*it <= 80) { best = *it; break; }
         b->H = best;
         for (sw2d_handle *h : run) {
             if (h->prm.tile_rows > 0 || h->H == best) continue;

@@ -112,7 +112,7 @@ struct Layout {
 
 class Search {
 public:
-    Search(const Counts &m, int n) : m(m), target(n) {}
+    Search(const Counts &counts, int n) : m(counts), target(n) {}
 
     // largest tile cost and the number of tiles with a positive cost
     // (Gorp.h:296-312)
