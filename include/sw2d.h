@@ -463,9 +463,11 @@ typedef struct sw2d_server_stats {
  * shares it among more requests: the same throughput, later results. */
 int sw2d_server_create(int device, int precision, size_t pool_bytes, int tile_rows, int slice_iters,
                        long max_active_cells, sw2d_server **out);
-/* Copies the request's arrays; returns its id.  Once a batch step has failed
- * the server stops: every running and queued request fails with the error
- * (sw2d_server_last_error) and submit returns SW2D_E_STATE. */
+/* Copies the request's arrays, the bed and initial depths already in the
+ * layout of the device arrays (on the calling thread, so that admission on the
+ * server's device thread is only the upload); returns its id.  Once a batch
+ * step has failed the server stops: every running and queued request fails
+ * with the error (sw2d_server_last_error) and submit returns SW2D_E_STATE. */
 int sw2d_server_submit(sw2d_server *s, const sw2d_request *req, long *id);
 int sw2d_server_poll(sw2d_server *s, long id, sw2d_result *out);
 /* (t, q) pairs of a finished request: 2*min(n, cap) doubles. */

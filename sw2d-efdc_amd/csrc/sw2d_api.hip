@@ -294,12 +294,43 @@ const char *sw2d_last_error(const sw2d_handle *h)
 
 namespace sw2d {
 
+void prepare_images(const sw2d_grid_desc *grid, const double *z, const double *h0, int precision,
+                    HostImages &img)
+{
+    const int R = grid->rows, C = grid->cols;
+    const size_t RC = (size_t)R * C;
+    const double nodata = grid->nodata;
+    double v0 = 0.0;
+    for (size_t i = 0; i < RC; i++)
+        if (z[i] != nodata) v0 += h0[i];
+    img.volume0 = v0 * grid->cellsize * grid->cellsize;
+    img.P = round_up(C, 16);
+    const size_t n = (size_t)R * img.P;
+    img.z.assign(n, 0.0);
+    for (int r = 0; r < R; r++)
+        for (int j = 0; j < C; j++) {
+            const double zv = z[(size_t)r * C + j];
+            img.z[(size_t)r * img.P + j] = (zv != nodata) ? zv : NAN;
+        }
+    img.h64.clear();
+    img.h32.clear();
+    if (depth_bytes(precision) == 8) {
+        img.h64.assign(n, 0.0);
+        for (int r = 0; r < R; r++)
+            for (int j = 0; j < C; j++) img.h64[(size_t)r * img.P + j] = h0[(size_t)r * C + j];
+    } else {
+        img.h32.assign(n, 0.0f);
+        for (int r = 0; r < R; r++)
+            for (int j = 0; j < C; j++) img.h32[(size_t)r * img.P + j] = (float)h0[(size_t)r * C + j];
+    }
+}
+
 int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
-                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out)
+                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out, const HostImages *img)
 {
     if (!out) return fail(nullptr, SW2D_E_ARG, "out is NULL");
     *out = nullptr;
-    if (!grid || !z || !h0 || !params) return fail(nullptr, SW2D_E_ARG, "NULL argument");
+    if (!grid || (!img && (!z || !h0)) || !params) return fail(nullptr, SW2D_E_ARG, "NULL argument");
     if (grid->rows < 2 || grid->cols < 2) return fail(nullptr, SW2D_E_ARG, "grid must be at least 2x2");
     if (!(grid->cellsize > 0)) return fail(nullptr, SW2D_E_ARG, "cellsize must be positive");
     if (!known_precision(params->precision)) return fail(nullptr, SW2D_E_ARG, "unknown precision");
@@ -333,6 +364,10 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
         if (params->precision != batch->precision)
             return fail(nullptr, SW2D_E_ARG, "member precision differs from the batch's");
     }
+    if (img && (params->kernel != SW2D_KERNEL_FUSED || img->P != round_up(grid->cols, 16) ||
+                img->z.size() != (size_t)grid->rows * img->P ||
+                (depth_bytes(params->precision) == 8 ? img->h64.size() : img->h32.size()) != img->z.size()))
+        return fail(nullptr, SW2D_E_ARG, "prepared images do not match the grid and precision");
 
     sw2d_handle *h = new sw2d_handle();
     h->graph_on = sw2d::graphs_enabled();
@@ -359,15 +394,14 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
 
     const int R = h->R, C = h->C;
     const size_t RC = (size_t)R * C;
-    const double nodata = grid->nodata;
-
-    // initial volume
-    double v0 = 0.0;
-    for (size_t i = 0; i < RC; i++)
-        if (z[i] != nodata) v0 += h0[i];
-    h->volume0 = v0 * grid->cellsize * grid->cellsize;
 
     if (fused(h)) {
+        HostImages own;
+        if (!img) {
+            prepare_images(grid, z, h0, params->precision, own);
+            img = &own;
+        }
+        h->volume0 = img->volume0;
         h->P = round_up(C, 16);
         const size_t n = (size_t)R * h->P;
         h->W = 58;
@@ -385,26 +419,17 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
         if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
         if ((rc = dalloc(h, &h->dscen, 1))) return bail(rc);
         // pitched images: inactive cells carry NaN in z, which stays FP64
-        std::vector<double> zi(n, 0.0);
-        for (int r = 0; r < R; r++)
-            for (int j = 0; j < C; j++) {
-                const double zv = z[(size_t)r * C + j];
-                zi[(size_t)r * h->P + j] = (zv != nodata) ? zv : NAN;
-            }
-        e = hipMemcpy(h->z, zi.data(), n * 8, hipMemcpyHostToDevice);
-        if (hb == 8) {
-            std::vector<double> hi(n, 0.0);
-            for (int r = 0; r < R; r++)
-                for (int j = 0; j < C; j++) hi[(size_t)r * h->P + j] = h0[(size_t)r * C + j];
-            if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 8, hipMemcpyHostToDevice);
-        } else {
-            std::vector<float> hi(n, 0.0f);
-            for (int r = 0; r < R; r++)
-                for (int j = 0; j < C; j++) hi[(size_t)r * h->P + j] = (float)h0[(size_t)r * C + j];
-            if (e == hipSuccess) e = hipMemcpy(h->h[0], hi.data(), n * 4, hipMemcpyHostToDevice);
-        }
+        e = hipMemcpy(h->z, img->z.data(), n * 8, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hb == 8 ? hipMemcpy(h->h[0], img->h64.data(), n * 8, hipMemcpyHostToDevice)
+                        : hipMemcpy(h->h[0], img->h32.data(), n * 4, hipMemcpyHostToDevice);
         if (e != hipSuccess) { hip_fail(h, e, "upload"); return bail(SW2D_E_HIP); }
     } else {
+        const double nodata = grid->nodata;
+        double v0 = 0.0;  // initial volume
+        for (size_t i = 0; i < RC; i++)
+            if (z[i] != nodata) v0 += h0[i];
+        h->volume0 = v0 * grid->cellsize * grid->cellsize;
         h->N = R * C + C / 2;
         h->NA = h->N + 2 * C + 4;
         double **arrs[] = {&h->pz, &h->ph, &h->pho, &h->phm, &h->phn, &h->pum, &h->pumo, &h->puu1,

@@ -109,8 +109,24 @@ std::vector<int> round_fill_heights(long slots, const std::function<long(int)> &
 void set_tiles(sw2d_handle *h, int H);
 int retile(sw2d_handle *h, int H);
 void advance(sw2d_handle *h, long m);
+// Host images of a fused-kernel scenario's inputs in the layout of its
+// device arrays: the bed in FP64 with NaN for inactive cells and the initial
+// depths in the form's depth type, both at pitch P (columns rounded up to
+// 16), plus the initial volume.  create_impl builds them itself unless it is
+// given them: a server builds them on the submitting thread, so that its
+// device thread, which keeps the GPU fed, only copies them.
+struct HostImages {
+    int P = 0;
+    double volume0 = 0;           // sum of h0 over active cells (row-major order), x dx^2
+    std::vector<double> z;        // R x P
+    std::vector<double> h64;      // R x P, FP64 depth forms
+    std::vector<float> h32;       // R x P, FP32 form
+};
+void prepare_images(const sw2d_grid_desc *grid, const double *z, const double *h0, int precision,
+                    HostImages &img);
 int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
-                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out);
+                const sw2d_params *params, sw2d_batch *batch, sw2d_handle **out,
+                const HostImages *img = nullptr);
 // defined in sw2d_output.hip
 void snapshot_free(sw2d_handle *h);
 // defined in sw2d_batch.hip

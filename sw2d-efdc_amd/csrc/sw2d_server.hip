@@ -34,7 +34,8 @@ namespace {
 struct Request {
     long id = 0;
     sw2d_request req{};
-    std::vector<double> z, h0, rain, mtab;
+    sw2d::HostImages img;        // bed and initial depths, built at submission
+    std::vector<double> rain, mtab;
     std::vector<unsigned char> mcls;
     sw2d_result res{};
     std::vector<double> hydro;   // (t, q) pairs after completion
@@ -107,8 +108,9 @@ struct sw2d_server {
         p.kernel = SW2D_KERNEL_FUSED;
         p.device = device;
         sw2d_handle *h = nullptr;
-        int rc = sw2d_batch_new(batch, &q.grid, r->z.data(), r->h0.data(), &p, &h);
-        if (rc) { r->err = sw2d_batch_last_error(batch); return rc; }
+        int rc = sw2d::create_impl(&q.grid, nullptr, nullptr, &p, batch, &h, &r->img);
+        if (rc) { r->err = sw2d::g_create_error; return rc; }
+        r->img = sw2d::HostImages();  // on the device now
         auto bad = [&](int code) {
             r->err = sw2d_last_error(h);
             sw2d_destroy(h);
@@ -368,9 +370,15 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
     auto r = std::make_unique<Request>();
     r->req = *q;
     const size_t RC = (size_t)R * C;
-    r->z.assign(q->z, q->z + RC);
-    if (q->h0) r->h0.assign(q->h0, q->h0 + RC);
-    else r->h0.assign(RC, 0.0);
+    // the device arrays' host images are built here, on the submitting
+    // thread: admission then only copies them, and the device thread goes
+    // back to stepping sooner
+    if (q->h0) {
+        sw2d::prepare_images(&q->grid, q->z, q->h0, s->precision, r->img);
+    } else {
+        const std::vector<double> zero(RC, 0.0);
+        sw2d::prepare_images(&q->grid, q->z, zero.data(), s->precision, r->img);
+    }
     if (q->nrain > 0) r->rain.assign(q->rain_mm, q->rain_mm + q->nrain);
     if (q->nclass > 0) {
         r->mcls.assign(q->manning_class, q->manning_class + RC);
