@@ -427,7 +427,8 @@ typedef struct sw2d_request {
 
 typedef struct sw2d_result {
     int state;                       /* SW2D_REQ_* */
-    double t_submit, t_admit, t_done;  /* seconds on the server's clock */
+    double t_submit, t_admit, t_done;  /* seconds on the server's clock; t_submit is
+                                        * the submit call's start */
     long iterations;
     double time;                     /* simulated time reached */
     long hydro_n;                    /* hydrograph records held */

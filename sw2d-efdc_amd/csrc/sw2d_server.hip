@@ -367,6 +367,8 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->dead) return SW2D_E_STATE;   // err holds the step failure
     }
+    // a request's latency counts from the call, its image building included
+    const double t_call = s->now();
     auto r = std::make_unique<Request>();
     r->req = *q;
     const size_t RC = (size_t)R * C;
@@ -400,7 +402,7 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
     std::lock_guard<std::mutex> lk(s->mu);
     r->id = s->next_id++;
     r->res.state = SW2D_REQ_QUEUED;
-    r->res.t_submit = s->now();
+    r->res.t_submit = t_call;
     if (id) *id = r->id;
     s->queue.push_back(r.get());
     s->all[r->id] = std::move(r);
