@@ -162,6 +162,20 @@ def test_bad_rank_count_is_reported(tmp_path):
     assert r.returncode == 1 and "at least 1" in r.stdout
 
 
+def test_malformed_cell_inp_is_reported(tmp_path):
+    """A CELL.INP without the title line, or one that ends inside a block of
+    rows, is an error with a message, not a partition of garbage."""
+    (tmp_path / "A.INP").write_text("no title here\nC\nC\nC\n")
+    t = np.full((6, 9), 5)
+    write_cell(tmp_path / "B.INP", t)
+    lines = (tmp_path / "B.INP").read_text().splitlines()
+    (tmp_path / "B.INP").write_text("\n".join(lines[:7]) + "\n")   # 3 of 6 rows
+    for name, msg in (("A.INP", "title"), ("B.INP", "ends inside a block")):
+        r = subprocess.run([str(DRIVER), "--lorp", str(tmp_path / name), "2", "3", str(tmp_path / "L")],
+                           capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and msg in r.stdout, (name, r.stdout)
+
+
 @pytest.mark.skipif(not GORP.exists(), reason="oracle/_ref/gorp not built (needs the reference tree)")
 def test_lorp_equals_the_reference_partitioner(tmp_path):
     rng = np.random.default_rng(11)
