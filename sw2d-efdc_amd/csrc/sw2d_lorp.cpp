@@ -24,9 +24,10 @@
 // nh = N/nv..N+2 and nh nv >= N, keeping the first lowest score.
 //
 // Where the reference rescans a tile's cells for every candidate move, the
-// counts here come from summed-area tables, so a score costs O(nh nv).  The
-// scores are the same doubles (integer counts, the same operations in the
-// same order), so every comparison, and the file, comes out the same.
+// counts here come from summed-area tables and a candidate is scored from the
+// tiles it changes (O(nh + nv) per move).  The scores are the same doubles
+// (integer counts, the same operations in the same order), so every
+// comparison, and the file, comes out the same.
 //
 // File (Gorp.h:674-739): its original name, a description line, the header
 // of the summary values, NH NV NActive Cost and the five weights (C++ stream
@@ -47,14 +48,14 @@ namespace sw2dio {
 namespace {
 
 // cost weights of the reference's model; the area terms are zero there
-const double kWeightTotalArea = 0.00 / 203 / 131;
-const double kWeightTileArea = 0.00 / 44 / 25;
-const double kWeightGhost = 0.001 / (4 * 44 + 4 * 25);
-const double kWeightLake = 0.007 / 44 / 25;
-const double kWeightLand = 0.00 / 44 / 25;
-const double kWeightBoundary = 0.003 / 44 / 25;
-const double kInvalid = 1e50;   // cost of a tile outside the grid or empty
-const double kPenalty = 1e10;   // per rank of difference in the active count
+constexpr double kWeightTotalArea = 0.00 / 203 / 131;
+constexpr double kWeightTileArea = 0.00 / 44 / 25;
+constexpr double kWeightGhost = 0.001 / (4 * 44 + 4 * 25);
+constexpr double kWeightLake = 0.007 / 44 / 25;
+constexpr double kWeightLand = 0.00 / 44 / 25;
+constexpr double kWeightBoundary = 0.003 / 44 / 25;
+constexpr double kInvalid = 1e50;   // cost of a tile outside the grid or empty
+constexpr double kPenalty = 1e10;   // per rank of difference in the active count
 const int kGhostCells = 4;      // added to every width in the file
 const int kFirstStep = 32;
 
@@ -157,25 +158,73 @@ public:
         return L;
     }
 
-    // the best single cut move by +-d (Gorp.h:579-635); true when one helped
+    // the best single cut move by +-d (Gorp.h:579-635); true when one helped.
+    // A move of cut i changes only the tiles on either side of it, so a
+    // candidate's score takes the other tiles' largest cost and active count
+    // from per-band maxima and counts of the current layout (the same doubles
+    // as a full evaluation: the total-area weight is zero, so that term adds
+    // +0.0 to a score whichever tiles are active).
     bool improve(Layout &L, int d) const
     {
-        double best = score(L.wx, L.wy);
-        std::vector<int> *axis = nullptr;
-        int cut = 0, step = 0;
-        for (std::vector<int> *v : {&L.wx, &L.wy}) {
-            for (int i = 0; i + 1 < (int)v->size(); i++) {
+        static_assert(kWeightTotalArea == 0.0, "incremental scores leave out the area term");
+        const int nh = (int)L.wx.size(), nv = (int)L.wy.size();
+        std::vector<int> X(nh + 1, 0), Y(nv + 1, 0);
+        for (int i = 0; i < nh; i++) X[i + 1] = X[i] + L.wx[i];
+        for (int r = 0; r < nv; r++) Y[r + 1] = Y[r] + L.wy[r];
+        // current tile costs, their maxima and active counts per column and row
+        std::vector<double> cmax(nh, 0.0), rmax(nv, 0.0);
+        std::vector<int> cact(nh, 0), ract(nv, 0);
+        int act = 0;
+        double top = 0;
+        for (int r = 0; r < nv; r++)
+            for (int i = 0; i < nh; i++) {
+                const double c = tile_cost(m, X[i], Y[r], L.wx[i], L.wy[r]);
+                if (c > 0) {
+                    act++; cact[i]++; ract[r]++;
+                    if (cmax[i] < c) cmax[i] = c;
+                    if (rmax[r] < c) rmax[r] = c;
+                    if (top < c) top = c;
+                }
+            }
+        auto penalised = [&](double c, int a) { return a != target ? c + kPenalty * abs(1 + a - target) : c; };
+        double best = penalised(top, act);
+        // the largest band maximum outside bands i and i+1
+        auto others = [](const std::vector<double> &mx, int i) {
+            double o = 0;
+            for (int k = 0; k < (int)mx.size(); k++)
+                if (k != i && k != i + 1 && o < mx[k]) o = mx[k];
+            return o;
+        };
+        int axis = -1, cut = 0, step = 0;
+        for (int ax = 0; ax < 2; ax++) {
+            const std::vector<int> &w = ax == 0 ? L.wx : L.wy;
+            const std::vector<double> &mx = ax == 0 ? cmax : rmax;
+            const std::vector<int> &na = ax == 0 ? cact : ract;
+            const std::vector<int> &P = ax == 0 ? X : Y;
+            const int nb = (int)w.size(), nc = ax == 0 ? nv : nh;
+            for (int i = 0; i + 1 < nb; i++) {
+                const double rest = others(mx, i);
                 for (int s : {d, -d}) {
-                    (*v)[i] += s; (*v)[i + 1] -= s;
-                    const double c = score(L.wx, L.wy);
-                    (*v)[i] -= s; (*v)[i + 1] += s;
-                    if (c < best) { best = c; axis = v; cut = i; step = s; }
+                    const int a0 = w[i] + s, a1 = w[i + 1] - s, p1 = P[i] + a0;
+                    double c = rest;
+                    int a = act - na[i] - na[i + 1];
+                    for (int k = 0; k < nc; k++) {
+                        const double t0 = ax == 0 ? tile_cost(m, P[i], Y[k], a0, L.wy[k])
+                                                  : tile_cost(m, X[k], P[i], L.wx[k], a0);
+                        const double t1 = ax == 0 ? tile_cost(m, p1, Y[k], a1, L.wy[k])
+                                                  : tile_cost(m, X[k], p1, L.wx[k], a1);
+                        if (t0 > 0) { a++; if (c < t0) c = t0; }
+                        if (t1 > 0) { a++; if (c < t1) c = t1; }
+                    }
+                    const double sc = penalised(c, a);
+                    if (sc < best) { best = sc; axis = ax; cut = i; step = s; }
                 }
             }
         }
-        if (!axis) return false;
-        (*axis)[cut] += step;
-        (*axis)[cut + 1] -= step;
+        if (axis < 0) return false;
+        std::vector<int> &w = axis == 0 ? L.wx : L.wy;
+        w[cut] += step;
+        w[cut + 1] -= step;
         return true;
     }
 
