@@ -67,7 +67,8 @@ def log(msg):
 
 def bytes_per_cell(precision, has_class_map=False):
     """Compulsory HBM bytes of one cell-update of the fused kernel: read the
-    depth h, the fluxes um, vn and the FP64 bed z, write h, um, vn (+1 byte
+    depth h, the fluxes um, vn and the bed (the FP64 z, or in the FP32 form
+    the cell's two FP32 bed steps: 8 B either way), write h, um, vn (+1 byte
     Manning class with a class map): 57 B in FP64, 33 B in the FP32 form,
     41 B in the mixed form (FP64 depth, FP32 fluxes)."""
     hb = {"fp64": 8, "fp32": 4, "mixed": 8}[precision]

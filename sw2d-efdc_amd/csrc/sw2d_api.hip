@@ -71,6 +71,7 @@ Scenario make_scenario(sw2d_handle *h)
         d.st[b][2] = h->vn[b];
     }
     d.z = (const double *)h->z;
+    d.dz = (const double *)h->dz;
     d.cls = h->cls;
     d.P = h->P; d.W = h->W; d.H = h->H;
     d.nstrips = h->nstrips; d.ntiles = h->ntiles;
@@ -417,12 +418,15 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
             else if ((rc = dalloc(h, (float **)b, n))) return bail(rc);
         }
         if ((rc = dalloc(h, (double **)&h->z, n))) return bail(rc);
+        // the FP32 form's kernel reads its bed steps (8 bytes a cell, as z)
+        if (params->precision == SW2D_FP32 && (rc = dalloc(h, (double **)&h->dz, n))) return bail(rc);
         if ((rc = dalloc(h, &h->dscen, 1))) return bail(rc);
         // pitched images: inactive cells carry NaN in z, which stays FP64
         e = hipMemcpy(h->z, img->z.data(), n * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hb == 8 ? hipMemcpy(h->h[0], img->h64.data(), n * 8, hipMemcpyHostToDevice)
                         : hipMemcpy(h->h[0], img->h32.data(), n * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess && h->dz) e = launch_bed_steps(h);
         if (e != hipSuccess) { hip_fail(h, e, "upload"); return bail(SW2D_E_HIP); }
     } else {
         const double nodata = grid->nodata;
@@ -1110,7 +1114,7 @@ int sw2d_device_bytes(sw2d_handle *h, long long *bytes)
     const long long RP = (long long)h->R * h->P;
     if (fused(h))
         b += 2 * RP * (depth_bytes(h->prm.precision) + 2 * flux_bytes(h->prm.precision)) + RP * 8 +
-             (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? RP : 0);
+             (h->dz ? RP * 8 : 0) + (h->scratch ? (long long)RC * 8 : 0) + (h->cls ? RP : 0);
     else b += (long long)16 * h->NA * 8;
     b += (long long)2 * h->nparts * sizeof(Partial);
     *bytes = b;

@@ -308,6 +308,7 @@ size_t sw2d_batch_footprint(const sw2d_grid_desc *grid, const sw2d_params *param
     size_t b = 0;
     b += 2 * pool_bytes_of(R * P * hb) + 4 * pool_bytes_of(R * P * fb);  // h, um, vn ping-pong
     b += pool_bytes_of(R * P * 8);                               // z
+    if (params->precision == SW2D_FP32) b += pool_bytes_of(R * P * 8);  // bed steps
     b += pool_bytes_of(sizeof(Scenario));                        // descriptor
     b += pool_bytes_of(3 * sizeof(Clock));
     b += pool_bytes_of(2 * ntiles * sizeof(Partial));

@@ -256,7 +256,10 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
     const int jc = colin ? j : (j < 0 ? 0 : C - 1);
     const Arr<const TH, kBuf> h_in(d.st[ib][0]);
     const Arr<const T, kBuf> um_in(d.st[ib][1]), vn_in(d.st[ib][2]);
-    const Arr<const double, kBuf> zg(d.z);
+    // the bed: FP64 elevations, or in the FP32 form the cell's bed-step pair
+    // (8 bytes as well; read as one double's bits).  Ghosts read NaN either way.
+    const Arr<const double, kBuf> zg(kH32 ? d.dz : d.z);
+    const double zghost = kH32 ? __longlong_as_double(0x7FC000007FC00000LL) : __builtin_nan("");
     auto load_row = [&](int k, TH &hh, T &uu, T &vv, double &zz, uint8_t &cc) {
         const bool in = colin && k >= 0 && k < R;
         const int kc = k < 0 ? 0 : (k >= R ? R - 1 : k);
@@ -267,7 +270,7 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         hh = in ? lh : TH(0);
         uu = in ? lu : T(0);
         vv = in ? lv : T(0);
-        zz = in ? lz : __builtin_nan("");
+        zz = in ? lz : zghost;
         cc = (map_on && in) ? clsg[off] : (uint8_t)0;
     };
     TH nh;
@@ -282,7 +285,12 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const double z0 = nz;
         const uint8_t cl0 = ncl;
         load_row(k + 1, nh, nu, nv, nz, ncl);
-        const bool a0 = (z0 == z0);  // inactive cells and ghosts carry NaN
+        // FP32 form: z0 holds the bed steps of row k's cell to its north (low
+        // word) and west (high word) neighbours, NaN for an inactive cell
+        const u32x2 zsteps = __builtin_bit_cast(u32x2, z0);
+        const T zstep_n = (T)__uint_as_float(zsteps.x), zstep_w = (T)__uint_as_float(zsteps.y);
+        // inactive cells and ghosts carry NaN
+        const bool a0 = kH32 ? (zstep_w == zstep_w) : (z0 == z0);
         // west/east neighbours of rows k-1 and k-2, made here rather than
         // carried from the row before (shorter register live ranges)
         const TH h1w = from_west(h1);
@@ -299,8 +307,8 @@ fused_step(const Scenario *__restrict__ scen, const TileRef *__restrict__ tiles,
         const bool a0w = west_bit(a0, lane);
         T dzn0 = 0, dzw0 = 0;
         if constexpr (kH32) {
-            dzn0 = (T)(z1 - z0);   // bed step across the north face of row k
-            dzw0 = (T)(z0w - z0);  // bed step across the west face of row k
+            dzn0 = zstep_n;  // bed step across the north face of row k: (float)(z(k-1) - z(k))
+            dzw0 = zstep_w;  // bed step across the west face of row k: (float)(z(j-1) - z(j))
         }
         const TH hn0d = (j == 0) ? h0 : TH(0.50) * (h0 + h0w);
         const TH hm0d = (k == 0) ? h0 : ((k == R) ? h1 : TH(0.50) * (h0 + h1));

@@ -129,6 +129,9 @@ struct Scenario {
     StepCommon s;            // slot_*, step_index and part_in/out are filled per launch
     void *st[2][3];          // (h, um, vn) of ping-pong buffer 0 and 1; iteration k reads k%2
     const double *z;         // bed elevation (FP64 in both forms); NaN marks an inactive cell
+    const double *dz;        // FP32 form: per cell its FP32 bed steps from the north and the
+                             // west neighbour (two floats in 8 bytes, NaN for an inactive
+                             // cell), which its fused kernel reads instead of z
     const uint8_t *cls;      // Manning class per cell (ncls_on)
     int P;                   // row pitch in elements
     int W;                   // output columns per tile (64 - 6)
@@ -194,6 +197,7 @@ struct sw2d_handle {
     // fused-path state (ping-pong)
     void *h[2] = {nullptr, nullptr}, *um[2] = {nullptr, nullptr}, *vn[2] = {nullptr, nullptr};
     void *z = nullptr;
+    void *dz = nullptr;                 // FP32 form: bed steps (R x P pairs of floats)
     uint8_t *cls = nullptr;
     int cur = 0;
     int W = 58, H = 64, nstrips = 0, nchunks = 0, ntiles = 0;
@@ -306,6 +310,7 @@ hipError_t launch_fused_mix(hipStream_t stream, const Scenario *scen, const Tile
                             int phase, int features, size_t lds = 0);
 hipError_t launch_init_f64(sw2d_handle *h);
 hipError_t launch_init_f32(sw2d_handle *h);
+hipError_t launch_bed_steps(sw2d_handle *h);
 hipError_t launch_init_mix(sw2d_handle *h);
 // one derived field of cells first .. first + count - 1 (row-major, rows x
 // cols) into out_dev[0 .. count - 1]
