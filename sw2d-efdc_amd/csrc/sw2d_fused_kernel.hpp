@@ -6,9 +6,11 @@
 // iterations is only (h, um, vn) (bed z is constant); every other array is a
 // local function of it.  This kernel reads (h, um, vn, z) once and writes
 // (h, um, vn) once per iteration: 56 B/cell in FP64; 32 B/cell in the FP32
-// form, whose bed elevation stays FP64 (it is input data: rounding a DEM to
-// binary32 moves it by up to half an ulp of its relief, which alone shifts
-// depths by ~1 mm on a 2.5 m-relief test DEM, while FP32 state costs 0.2 mm).
+// form, whose bed is not rounded to binary32 (it is input data: rounding a
+// DEM to binary32 moves it by up to half an ulp of its relief, which alone
+// shifts depths by ~1 mm on a 2.5 m-relief test DEM, while FP32 state costs
+// 0.2 mm): it reads, in the same 8 bytes, the cell's bed steps to its north
+// and west neighbours, the FP64 differences rounded once at creation.
 //
 // Work decomposition.  A tile is one wavefront (64 lanes) and covers 58
 // output columns x H output rows.  Lane l holds column j = j0 - 3 + l: the
