@@ -203,6 +203,32 @@ def test_fp32_precision_form(pkg, po, cases):
     assert abs(b["error"]) <= 1e-6 * b["volume"]
 
 
+def test_fp32_precision_form_with_holes(pkg, po, cases):
+    """The FP32 form on a bed with NODATA cells (scattered, two edge columns
+    and a row segment): inactive cells stay dry with no flow across their
+    faces, and the depths follow the FP64 oracle within 1 mm.  The FP32 form
+    reads per-cell bed steps whose NaN marks an inactive cell and whose value
+    toward an inactive neighbour is never used."""
+    R, C = 61, 83
+    z = dem(cases, R, C, 11, holes=True)
+    h0 = np.zeros((R, C))
+    g = pkg.Case(z, h0, 10.0, dt=0.5, manning=0.03, precision="fp32")
+    o = po.Oracle(z, h0, 10.0, dt=0.5, manning=0.03)
+    rain = np.array([60.0, 30.0, 0.0])
+    for c in (g, o):
+        c.set_rain(rain, 300.0)
+    g.step(900)
+    o.step(900)
+    hole = z == -9999.0
+    h = g.field("h")
+    assert np.all(h[hole] == 0.0)
+    assert np.count_nonzero(o.field("h") > 1e-3) > 200   # the run wetted the domain
+    dh = np.abs(h - o.field("h"))
+    assert dh.max() <= 1e-3, dh.max()
+    b = g.balance()
+    assert abs(b["error"]) <= 1e-6 * b["volume"]
+
+
 def test_autotune_keeps_results(pkg, po, cases):
     """The measured tile height changes the schedule, never the numbers."""
     R, C = 90, 140
