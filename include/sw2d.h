@@ -32,7 +32,7 @@ extern "C" {
 #define SW2D_OK 0
 #define SW2D_E_ARG (-1)     /* bad argument */
 #define SW2D_E_HIP (-2)     /* HIP runtime error */
-#define SW2D_E_NOMEM (-3)   /* device allocation failed */
+#define SW2D_E_NOMEM (-3)   /* device (or host) allocation failed */
 #define SW2D_E_STATE (-4)   /* call not valid in this state */
 #define SW2D_E_IO (-5)      /* file missing or malformed */
 #define SW2D_E_NODEV (-6)   /* no usable GPU */

@@ -399,7 +399,12 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
     if (fused(h)) {
         HostImages own;
         if (!img) {
-            prepare_images(grid, z, h0, params->precision, own);
+            try {
+                prepare_images(grid, z, h0, params->precision, own);
+            } catch (const std::bad_alloc &) {
+                fail(h, SW2D_E_NOMEM, "host memory for the input images");
+                return bail(SW2D_E_NOMEM);
+            }
             img = &own;
         }
         h->volume0 = img->volume0;
@@ -441,7 +446,13 @@ int create_impl(const sw2d_grid_desc *grid, const double *z, const double *h0,
         for (double **a : arrs)
             if ((rc = dalloc(h, a, h->NA))) return bail(rc);
         if ((rc = dalloc(h, &h->pinf, h->NA))) return bail(rc);
-        std::vector<int> inf(h->NA, 0);
+        std::vector<int> inf;
+        try {
+            inf.assign(h->NA, 0);
+        } catch (const std::bad_alloc &) {
+            fail(h, SW2D_E_NOMEM, "host memory for the activity mask");
+            return bail(SW2D_E_NOMEM);
+        }
         for (size_t i = 0; i < RC; i++) inf[i] = (z[i] != nodata) ? 1 : 0;
         e = hipMemcpy(h->pz, z, RC * 8, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(h->ph, h0, RC * 8, hipMemcpyHostToDevice);

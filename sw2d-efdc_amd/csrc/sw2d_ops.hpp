@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <functional>
+#include <new>
 #include <string>
 #include <vector>
 

@@ -375,11 +375,17 @@ int sw2d_server_submit(sw2d_server *s, const sw2d_request *q, long *id)
     // the device arrays' host images are built here, on the submitting
     // thread: admission then only copies them, and the device thread goes
     // back to stepping sooner
-    if (q->h0) {
-        sw2d::prepare_images(&q->grid, q->z, q->h0, s->precision, r->img);
-    } else {
-        const std::vector<double> zero(RC, 0.0);
-        sw2d::prepare_images(&q->grid, q->z, zero.data(), s->precision, r->img);
+    try {
+        if (q->h0) {
+            sw2d::prepare_images(&q->grid, q->z, q->h0, s->precision, r->img);
+        } else {
+            const std::vector<double> zero(RC, 0.0);
+            sw2d::prepare_images(&q->grid, q->z, zero.data(), s->precision, r->img);
+        }
+    } catch (const std::bad_alloc &) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "host memory for the request's images";
+        return SW2D_E_NOMEM;
     }
     if (q->nrain > 0) r->rain.assign(q->rain_mm, q->rain_mm + q->nrain);
     if (q->nclass > 0) {
