@@ -313,8 +313,10 @@ def batch_leg(pkg, cases, n, plain_sample):
     samp = [(specs[i].z.shape[0], plain_seconds(pkg, specs[i]) / (specs[i].cells * iters[i]))
             for i in idx]
     sides = np.array([s for s, _ in samp], dtype=float)
-    t_plain = 0.0
+    t_plain = 0.0 if samp else None   # no sample: no plain comparison
     for s, k in zip(specs, iters):
+        if not samp:
+            break
         j = int(np.argmin(np.abs(np.log(sides / s.z.shape[0]))))
         t_plain += samp[j][1] * s.cells * k
     log(f"batch: plain fp64 sample of {len(idx)} scenarios measured")
@@ -345,7 +347,7 @@ def batch_leg(pkg, cases, n, plain_sample):
         "plain_fp64_sample": {"scenarios": len(idx), "sides": [int(s) for s, _ in samp],
                               "ns_per_cell_update": [1e9 * v for _, v in samp]},
         "plain_fp64_one_at_a_time_s": t_plain,
-        "scenario_throughput_vs_plain_fp64": t_plain / wall,
+        "scenario_throughput_vs_plain_fp64": t_plain / wall if t_plain is not None else None,
     }
 
 
