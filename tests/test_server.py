@@ -228,11 +228,13 @@ def test_adaptive_tiling_keeps_results(pkg, cases):
     srv.close()
 
 
-def test_fp64_server_with_adaptive_tiling_equals_alone(pkg, cases):
-    """The FP64 (parity) form served with adaptive tiling: each hydrograph
-    equals the stand-alone FP64 run's value for value."""
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_server_with_adaptive_tiling_equals_alone(pkg, cases, precision):
+    """The FP64 (parity) form and the mixed form (FP64 depths, FP32 fluxes)
+    served with adaptive tiling: each hydrograph equals the stand-alone run's
+    in the same form value for value."""
     reqs = [request(cases, i) for i in (0, 1, 2, 5)]
-    srv = pkg.Server(precision="fp64", pool_bytes=256 << 20, tile_rows=0, slice_iters=12,
+    srv = pkg.Server(precision=precision, pool_bytes=256 << 20, tile_rows=0, slice_iters=12,
                      max_active_cells=2 * max(z.size for z, *_ in reqs))
     ids = [srv.submit(z, 10.0, t_end=t_end, rain=rain, dtrain=900.0, outlet=outlet,
                       hydro_cap=CAP, **kw) for z, outlet, rain, kw, t_end in reqs]
@@ -240,7 +242,7 @@ def test_fp64_server_with_adaptive_tiling_equals_alone(pkg, cases):
     for i, (z, outlet, rain, kw, t_end) in enumerate(reqs):
         r = srv.poll(ids[i])
         assert r["state"] == "done"
-        c = pkg.Case(z, np.zeros_like(z), 10.0, precision="fp64", **kw)
+        c = pkg.Case(z, np.zeros_like(z), 10.0, precision=precision, **kw)
         c.set_rain(rain, 900.0)
         c.set_outlet(*outlet)
         c.set_hydrograph(CAP)
