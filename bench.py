@@ -713,7 +713,7 @@ def main():
     ap.add_argument("--serve-precision", default="fp32", choices=["fp32", "mixed", "fp64"])
     ap.add_argument("--serve-tile-rows", type=int, default=0,
                     help="server tile height (0 = adaptive: retiled to the running set)")
-    ap.add_argument("--serve-active-cells", type=int, default=32 << 20,
+    ap.add_argument("--serve-active-cells", type=int, default=64 << 20,
                     help="server admission cap on running cells (0 = HBM footprint only)")
     ap.add_argument("--serve-err-sample", type=int, default=8,
                     help="requests of the stream rerun in FP64 for the errors (0 = skip)")
